@@ -1,0 +1,314 @@
+"""Benchmark of the P2G/G2P hot path on one MI355X.
+
+Workload (128^3 periodic cells, cell size 1, 16 particles per cell =
+33,554,432 particles, quadratic splines), one step =
+  forward:  fused P2G of mass (ScalarValue), momentum (VectorValue) and stress
+            (TensorDivergence, multiplier -V) + ghost fold;
+            fused G2P of a grid velocity field (VectorValue + VectorGradient)
+            after a ghost fill;
+  adjoint:  G2P adjoint (grid-velocity and position cotangents, P2G-shaped
+            scatter) and P2G adjoint (mass/momentum/stress and position
+            cotangents, G2P-shaped gather).
+Particles are cell-sorted once at setup with the library's LinkedCellList
+build + permute (inputs resident in HBM before the timed region); the
+per-step sort is timed as a separate figure.  The headline value is the
+FP32-mixed forward+adjoint rate; the FP64 plain path runs the same step.
+
+Prints ONE JSON line.  --gpus N > 1 exits: this build measures one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NODES = 131 ** 3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=128)
+    ap.add_argument("--ppc", type=int, default=16)
+    ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--mode", choices=["both", "fp32", "fp64"], default="both")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(kernel, n, s, nodes):
+    """Minimum HBM bytes of one launch: every particle input read once, every
+    accumulated (+=) output read and written once, positions FP64 (24 B),
+    position cotangents FP64 (RMW, 48 B), grid arrays read (or RMW) once."""
+    if kernel.startswith("p2g_mpm"):
+        return n * (24 + 13 * s) + nodes * 7 * s * 2
+    if kernel.startswith("g2p_mpm"):
+        return n * (24 + 2 * 12 * s) + nodes * 3 * s
+    if kernel.startswith("g2p_adjoint_mpm"):
+        return n * (24 + 12 * s + 48) + nodes * 3 * s + nodes * 3 * s * 2
+    if kernel.startswith("p2g_adjoint_mpm"):
+        return n * (24 + 13 * s + 2 * 13 * s + 48) + nodes * 7 * s
+    return None
+
+
+class Workload:
+    def __init__(self, torch, ca, dtype, cells, ppc, order, seed=20260102):
+        self.torch = torch
+        self.ca = ca
+        self.dtype = dtype
+        self.order = order
+        dev = torch.device("cuda", 0)
+        n = cells ** 3 * ppc
+        self.n = n
+        L = float(cells)
+        gm = ca.createUniformGlobalMesh([0.0] * 3, [L] * 3, 1.0)
+        self.lg = ca.createLocalGrid(ca.createGlobalGrid(gm, [True] * 3), 1)
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        pos = torch.rand(n, 3, dtype=torch.float64, device=dev, generator=g) * L
+        # Cell-sort once (LinkedCellList over the mesh cells + permute).
+        self.lcl = ca.createLinkedCellList(pos, [1.0] * 3, [0.0] * 3, [L] * 3)
+        ca.permute(self.lcl, pos)
+        self.pos = pos
+        mass = (torch.rand(n, dtype=torch.float64, device=dev, generator=g) + 0.5)
+        vel = torch.randn(n, 3, dtype=torch.float64, device=dev, generator=g)
+        s6 = torch.randn(n, 6, dtype=torch.float64, device=dev, generator=g)
+        sig = torch.stack([s6[:, 0], s6[:, 3], s6[:, 4], s6[:, 3], s6[:, 1], s6[:, 5],
+                           s6[:, 4], s6[:, 5], s6[:, 2]], dim=1).reshape(n, 3, 3)
+        self.m = mass.to(dtype)
+        self.q = (mass[:, None] * vel).to(dtype)
+        self.sig = sig.to(dtype)
+        del vel, s6, sig, mass
+        arr = lambda d: ca.createArray("g", ca.createArrayLayout(self.lg, d, ca.Node()),  # noqa
+                                       dtype=dtype)
+        self.gmass, self.gmom, self.gstress = arr(1), arr(3), arr(3)
+        self.halo = ca.createHalo(ca.NodeHaloPattern(), 1, self.gmass)
+        # Prescribed grid velocity u_d = sin(2 pi x_d / L) on the nodes.
+        self.u = arr(3)
+        x = (torch.arange(cells + 3, dtype=torch.float64, device=dev) - 1.0)
+        s = torch.sin(2 * math.pi * x / L)
+        uv = self.u.view()
+        uv[..., 0] = s[:, None, None]
+        uv[..., 1] = s[None, :, None]
+        uv[..., 2] = s[None, None, :]
+        self.vel = torch.zeros(n, 3, dtype=dtype, device=dev)
+        self.L = torch.zeros(n, 3, 3, dtype=dtype, device=dev)
+        # Adjoint seeds and outputs.
+        self.ubar_v = torch.randn(n, 3, dtype=torch.float64, device=dev, generator=g).to(dtype)
+        self.ubar_L = torch.randn(n, 3, 3, dtype=torch.float64, device=dev,
+                                  generator=g).to(dtype)
+        self.gbar_u = arr(3)
+        self.gbar_m, self.gbar_q, self.gbar_s = arr(1), arr(3), arr(3)
+        for a in (self.gbar_m, self.gbar_q, self.gbar_s):
+            a.view().copy_(torch.randn(a.view().shape, dtype=torch.float64, device=dev,
+                                       generator=g).to(dtype))
+        self.mbar = torch.zeros(n, dtype=dtype, device=dev)
+        self.qbar = torch.zeros(n, 3, dtype=dtype, device=dev)
+        self.sbar = torch.zeros(n, 3, 3, dtype=dtype, device=dev)
+        self.xbar = torch.zeros(n, 3, dtype=torch.float64, device=dev)
+        self.sp = ca.Spline(order)
+        self.p2g_f = [ca.createScalarValueP2G(self.m, 1.0), ca.createVectorValueP2G(self.q, 1.0),
+                      ca.createTensorDivergenceP2G(self.sig, -1.0)]
+        self.g2p_f = [ca.createVectorValueG2P(self.vel, 1.0),
+                      ca.createVectorGradientG2P(self.L, 1.0)]
+        self.g2p_adj_f = [ca.createVectorValueG2P(self.ubar_v, 1.0),
+                          ca.createVectorGradientG2P(self.ubar_L, 1.0)]
+
+    def step(self):
+        ca = self.ca
+        for a in (self.gmass, self.gmom, self.gstress, self.gbar_u):
+            a.view().zero_()
+        ca.p2g(self.p2g_f, self.pos, self.n, self.sp, self.halo,
+               [self.gmass, self.gmom, self.gstress], bins=self.lcl)
+        ca.g2p([self.u, self.u], self.halo, self.pos, self.n, self.sp, self.g2p_f,
+               bins=self.lcl)
+        ca.g2p_adjoint([self.u, self.u], self.halo, self.pos, self.n, self.sp, self.g2p_adj_f,
+                       [self.ubar_v, self.ubar_L], [self.gbar_u, self.gbar_u], self.xbar,
+                       bins=self.lcl)
+        ca.p2g_adjoint(self.p2g_f, self.pos, self.n, self.sp, self.halo,
+                       [self.gbar_m, self.gbar_q, self.gbar_s],
+                       [self.mbar, self.qbar, self.sbar], self.xbar, bins=self.lcl)
+
+    def sort_step(self):
+        """Per-step re-binning: LinkedCellList build + permute of the
+        positions and the 13 particle attributes."""
+        self.lcl.build(self.pos, 0, self.n)
+        self.ca.permute(self.lcl, self.pos, self.m, self.q, self.sig)
+
+
+def time_steps(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(steps):
+        fn()
+    t1.record()
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / steps
+
+
+def kernel_profile(native, torch, fn, steps):
+    """Average HIP-event duration of every kernel family over `steps` steps
+    (events are recorded on the launch stream around each launch)."""
+    import ctypes
+    lib = native.load()
+    ctx = native.context()
+    native.check(lib.cabana_amd_profile_reset(ctx), ctx)
+    native.check(lib.cabana_amd_profile_enable(ctx, 1), ctx)
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    cnt = ctypes.c_int32()
+    native.check(lib.cabana_amd_profile_count(ctx, ctypes.byref(cnt)), ctx)
+    out = {}
+    for i in range(cnt.value):
+        name = ctypes.create_string_buffer(64)
+        launches = ctypes.c_int64()
+        ms = ctypes.c_double()
+        native.check(lib.cabana_amd_profile_entry(ctx, i, name, 64, ctypes.byref(launches),
+                                                  ctypes.byref(ms)), ctx)
+        out[name.value.decode()] = {"launches": launches.value,
+                                    "avg_ms": ms.value / max(launches.value, 1),
+                                    "total_ms": ms.value}
+    native.check(lib.cabana_amd_profile_enable(ctx, 0), ctx)
+    return out
+
+
+def cpu_baseline(cells, ppc, order, budget_s):
+    """The oracle (OpenMP restatement of Cabana's p2g/g2p with ScatterView
+    duplication, plus OpenMP adjoints) on the host cores, on a bounded sample
+    of particles of the same workload; returns particles/s for fwd+adj."""
+    import numpy as np
+
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    threads = max(1, min(threads, 16))
+    L = float(cells)
+    mesh = oracle.OMesh([0.0] * 3, [L] * 3, 1.0, halo=1)
+    rng = np.random.default_rng(20260102)
+
+    def run(n):
+        pts = rng.uniform(0, L, size=(n, 3))
+        m = rng.uniform(0.5, 1.5, size=(n, 1))
+        q = rng.normal(size=(n, 3))
+        sig = rng.normal(size=(n, 9))
+        ub_v = rng.normal(size=(n, 3))
+        ub_l = rng.normal(size=(n, 9))
+        gm, gq, gs, u = mesh.grid(1), mesh.grid(3), mesh.grid(3), mesh.grid(3)
+        u[...] = rng.normal(size=u.shape)
+        vel = np.zeros((n, 3))
+        lg = np.zeros((n, 9))
+        gb_u = mesh.grid(3)
+        xbar = np.zeros((n, 3))
+        mb, qb, sb = np.zeros((n, 1)), np.zeros((n, 3)), np.zeros((n, 9))
+        gbm, gbq, gbs = mesh.grid(1), mesh.grid(3), mesh.grid(3)
+        t = time.perf_counter()
+        oracle.p2g(mesh, order, 0, 1.0, pts, m, gm, threads=threads)
+        oracle.p2g(mesh, order, 1, 1.0, pts, q, gq, threads=threads)
+        oracle.p2g(mesh, order, 4, -1.0, pts, sig, gs, threads=threads)
+        oracle.g2p(mesh, order, 1, 1.0, u, pts, vel, threads=threads)
+        oracle.g2p(mesh, order, 3, 1.0, u, pts, lg, threads=threads)
+        oracle.g2p_adjoint(mesh, order, 1, 1.0, u, pts, ub_v, gb_u, xbar, threads=threads)
+        oracle.g2p_adjoint(mesh, order, 3, 1.0, u, pts, ub_l, gb_u, xbar, threads=threads)
+        oracle.p2g_adjoint(mesh, order, 0, 1.0, pts, m, gbm, mb, xbar, threads=threads)
+        oracle.p2g_adjoint(mesh, order, 1, 1.0, pts, q, gbq, qb, xbar, threads=threads)
+        oracle.p2g_adjoint(mesh, order, 4, -1.0, pts, sig, gbs, sb, xbar, threads=threads)
+        return time.perf_counter() - t
+
+    n0 = 1 << 17
+    t0 = run(n0)
+    n = int(min(cells ** 3 * ppc, max(n0, n0 * budget_s / max(t0, 1e-3))))
+    t = run(n)
+    return {"value": n / t, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": "%d of %d particles, full 128^3 grid, FP64, same fwd+adj operator set "
+                      "(oracle: C restatement, OpenMP with per-thread grid copies), %.1f s"
+                      % (n, cells ** 3 * ppc, t)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only (this build has no multi-GPU path); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import cabana_amd as ca
+    from cabana_amd import _native
+    torch.cuda.set_device(0)
+    res = {}
+    modes = [("fp32", torch.float32), ("fp64", torch.float64)]
+    if args.mode != "both":
+        modes = [m for m in modes if m[0] == args.mode]
+    for label, dtype in modes:
+        w = Workload(torch, ca, dtype, args.cells, args.ppc, args.order)
+        ms = time_steps(torch, w.step, args.steps, args.warmup)
+        prof = kernel_profile(_native, torch, w.step, max(2, min(args.steps, 5)))
+        ms_sort = time_steps(torch, w.sort_step, 2, 1)
+        ca.check_domain()
+        res[label] = {"ms": ms, "rate": w.n / (ms * 1e-3), "kernels": prof, "n": w.n,
+                      "sort_ms": ms_sort}
+        print("[bench] %s: %.3f ms/step, %.3e particles/s, sort %.3f ms" %
+              (label, ms, w.n / (ms * 1e-3), ms_sort), file=sys.stderr, flush=True)
+        del w
+        torch.cuda.empty_cache()
+
+    head = res.get("fp32") or res.get("fp64")
+    s = 4 if "fp32" in res else 8
+    kernels = head["kernels"]
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"] /
+              max(kernels[k]["launches"], 1) * (1 if algorithmic_bytes(k, 1, s, 1) else 0))
+    ab = algorithmic_bytes(dom, head["n"], s, NODES)
+    achieved = ab / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "algorithmic_bytes": ab}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cells, args.ppc, args.order, args.cpu_seconds)
+    out = {
+        "metric": "P2G+G2P particles/s at 128^3x16ppc quadratic: FP32-mixed vs FP64, "
+                  "fwd+adjoint",
+        "value": head["rate"],
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mixed" if "fp32" in res else "f64",
+        "data": "synthetic (uniform random positions, seeded; cell-sorted at setup)",
+        "config": {"workload": "128^3 periodic cells x 16 ppc = %d particles, Spline<2>, "
+                               "P2G{mass,momentum,stress-div}+G2P{vel,vel-grad} fwd+adjoint"
+                               % head["n"],
+                   "cells": args.cells, "ppc": args.ppc, "order": args.order},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if "fp64" in res:
+        out["fp64"] = {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms"]}
+        if "fp32" in res:
+            out["fp32_over_fp64"] = res["fp32"]["rate"] / res["fp64"]["rate"]
+    out["sort_ms_per_step"] = head["sort_ms"]
+    out["value_with_sort"] = head["n"] / ((head["ms"] + head["sort_ms"]) * 1e-3)
+    out["kernels"] = {k: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"]}
+                      for k, v in kernels.items()}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,14 @@
+"""cabana_amd: MI355X-native particle<->grid interpolation with Cabana's
+p2g / g2p operator semantics.  The compute runs in hand-written HIP kernels
+(libcabana_amd.so); this package is the host-side mirror of the reference
+interface over the C ABI."""
+from .grid import (  # noqa: F401
+    AoSoASlice, Array, ArrayLayout, ArrayOp, Cell, Ghost, GlobalGrid, GlobalMesh, Halo,
+    IndexSpace, Local, LocalGrid, LocalMesh, Node, NodeHaloPattern, Own, ScatterReduce,
+    Spline, check_domain, createArray, createArrayLayout, createGlobalGrid, createHalo,
+    createLocalGrid, createLocalMesh, createScalarGradientG2P, createScalarGradientP2G,
+    createScalarValueG2P, createScalarValueP2G, createTensorDivergenceP2G,
+    createUniformGlobalMesh, createVectorDivergenceG2P, createVectorDivergenceP2G,
+    createVectorGradientG2P, createVectorValueG2P, createVectorValueP2G, g2p, g2p_adjoint,
+    grid_descriptor, p2g, p2g_adjoint, particle_descriptor)
+from .linkedcell import LinkedCellList, createLinkedCellList, permute  # noqa: F401

@@ -1,0 +1,1253 @@
+// C-ABI layer: descriptor validation, plan selection and kernel dispatch.
+#include "ca_internal.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace cabana_amd;
+
+namespace
+{
+
+thread_local std::string g_thread_error = "";
+
+struct Pending
+{
+    std::string name;
+    hipEvent_t start;
+    hipEvent_t stop;
+};
+
+struct Stat
+{
+    int64_t launches = 0;
+    double ms = 0.0;
+};
+
+} // namespace
+
+struct cabana_amd_context : public Launcher
+{
+    int device = 0;
+    std::string error;
+    unsigned long long* skipped = nullptr; // device counter
+    unsigned long long* bad = nullptr;     // device counter (binning)
+
+    // Reusable scratch.
+    void* scratch[4] = { nullptr, nullptr, nullptr, nullptr };
+    size_t scratch_bytes[4] = { 0, 0, 0, 0 };
+
+    // Kernel timing.
+    bool profiling = false;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, Stat> stats;
+    std::vector<std::string> order;
+
+    hipEvent_t get_event()
+    {
+        if ( !pool.empty() )
+        {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        hipEventCreate( &e );
+        return e;
+    }
+
+    void before( const char* name, hipStream_t s ) override
+    {
+        if ( !profiling )
+            return;
+        Pending p;
+        p.name = name;
+        p.start = get_event();
+        p.stop = nullptr;
+        hipEventRecord( p.start, s );
+        pending.push_back( p );
+    }
+
+    void after( const char* name, hipStream_t s ) override
+    {
+        if ( !profiling || pending.empty() )
+            return;
+        Pending& p = pending.back();
+        if ( p.name != name || p.stop )
+            return;
+        p.stop = get_event();
+        hipEventRecord( p.stop, s );
+    }
+
+    hipError_t drain()
+    {
+        hipError_t first = hipSuccess;
+        for ( auto& p : pending )
+        {
+            if ( p.stop )
+            {
+                hipError_t e = hipEventSynchronize( p.stop );
+                float ms = 0.0f;
+                if ( e == hipSuccess )
+                    e = hipEventElapsedTime( &ms, p.start, p.stop );
+                if ( e != hipSuccess && first == hipSuccess )
+                    first = e;
+                if ( !stats.count( p.name ) )
+                    order.push_back( p.name );
+                Stat& st = stats[p.name];
+                st.launches += 1;
+                st.ms += ms;
+                pool.push_back( p.stop );
+            }
+            pool.push_back( p.start );
+        }
+        pending.clear();
+        return first;
+    }
+
+    hipError_t ensure( int slot, size_t bytes )
+    {
+        if ( scratch_bytes[slot] >= bytes )
+            return hipSuccess;
+        if ( scratch[slot] )
+            hipFree( scratch[slot] );
+        scratch[slot] = nullptr;
+        scratch_bytes[slot] = 0;
+        size_t want = bytes + bytes / 4 + 256;
+        hipError_t e = hipMalloc( &scratch[slot], want );
+        if ( e == hipSuccess )
+            scratch_bytes[slot] = want;
+        return e;
+    }
+};
+
+namespace
+{
+
+enum Slot
+{
+    SLOT_CELL_OF = 0,
+    SLOT_CURSOR = 1,
+    SLOT_SCAN = 2,
+    SLOT_PERMUTE = 3
+};
+
+cabana_amd_status fail( cabana_amd_context* ctx, cabana_amd_status st,
+                        const std::string& msg )
+{
+    if ( ctx )
+        ctx->error = msg;
+    g_thread_error = msg;
+    return st;
+}
+
+cabana_amd_status hip_fail( cabana_amd_context* ctx, hipError_t e,
+                            const char* where )
+{
+    std::string msg = std::string( where ) + ": " + hipGetErrorString( e );
+    return fail( ctx, e == hipErrorOutOfMemory ? CABANA_AMD_ERROR_OUT_OF_MEMORY
+                                               : CABANA_AMD_ERROR_DEVICE,
+                 msg );
+}
+
+#define CA_HIP( ctx, expr, where )                                            \
+    do                                                                         \
+    {                                                                          \
+        hipError_t e__ = ( expr );                                             \
+        if ( e__ != hipSuccess )                                               \
+            return hip_fail( ctx, e__, where );                                \
+    } while ( 0 )
+
+#define CA_CHECK( st )                                                         \
+    do                                                                         \
+    {                                                                          \
+        cabana_amd_status s__ = ( st );                                        \
+        if ( s__ != CABANA_AMD_OK )                                            \
+            return s__;                                                        \
+    } while ( 0 )
+
+int dtype_size( int dtype )
+{
+    switch ( dtype )
+    {
+    case CABANA_AMD_FLOAT64:
+    case CABANA_AMD_INT64:
+        return 8;
+    case CABANA_AMD_FLOAT32:
+    case CABANA_AMD_INT32:
+        return 4;
+    default:
+        return 0;
+    }
+}
+
+// GlobalMesh + LocalMesh + evaluateSpline constants
+// (Cabana_Grid_GlobalMesh.hpp:59-80,160-165; Cabana_Grid_LocalMesh.hpp:89-104;
+// Cabana_Grid_Splines.hpp:1296-1318).
+cabana_amd_status make_mesh( cabana_amd_context* ctx,
+                             const cabana_amd_mesh* mesh, MeshDev& m )
+{
+    if ( !mesh )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "mesh is NULL" );
+    if ( mesh->halo_cell_width < 0 )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "negative halo cell width" );
+    for ( int d = 0; d < 3; ++d )
+    {
+        const double cell = mesh->cell_size[d];
+        const double extent = mesh->high_corner[d] - mesh->low_corner[d];
+        if ( !( cell > 0.0 ) || !( extent > 0.0 ) )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "mesh extent and cell size must be positive" );
+        const double nc = std::rint( extent / cell );
+        if ( nc < 1.0 || nc > 1.0e8 )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "mesh cell count out of range" );
+        const int n = (int)nc;
+        if ( std::fabs( n * cell - extent ) > 100.0 * 2.220446049250313e-16 )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "Extent not evenly divisible by uniform cell size" );
+        m.num_cell[d] = n;
+        m.num_node[d] = n + 2 * mesh->halo_cell_width + 1;
+        m.periodic[d] = mesh->periodic[d] ? 1 : 0;
+        const double own_low = mesh->low_corner[d] + cell * 0;
+        const double ghost_low = own_low - mesh->halo_cell_width * cell;
+        const double x0 = ghost_low + (double)0 * cell;
+        const double x1 = ghost_low + (double)1 * cell;
+        m.low[d] = x0;
+        m.rdx[d] = 1.0 / ( x1 - x0 );
+    }
+    m.halo = mesh->halo_cell_width;
+    return CABANA_AMD_OK;
+}
+
+cabana_amd_status make_pfield( cabana_amd_context* ctx,
+                               const cabana_amd_particle_field& f,
+                               int ncomp, int dtype, int64_t n_needed,
+                               bool allow_null, const char* what,
+                               PFieldDev& out )
+{
+    std::memset( &out, 0, sizeof( out ) );
+    if ( !f.data )
+    {
+        if ( allow_null )
+            return CABANA_AMD_OK;
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     std::string( what ) + ": data is NULL" );
+    }
+    if ( ncomp > 0 && f.num_comp != ncomp )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     std::string( what ) + ": expected " +
+                         std::to_string( ncomp ) + " components, got " +
+                         std::to_string( f.num_comp ) );
+    if ( f.num_comp < 1 )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     std::string( what ) + ": no components" );
+    if ( dtype >= 0 && f.dtype != dtype )
+        return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED,
+                     std::string( what ) + ": unsupported dtype" );
+    if ( f.num_particle < n_needed )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     std::string( what ) + ": fewer particles than points" );
+    if ( f.vector_length < 1 )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     std::string( what ) + ": vector_length < 1" );
+    if ( f.num_particle > f.vector_length &&
+         f.soa_stride < f.vector_length * f.num_comp )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     std::string( what ) +
+                         ": soa_stride smaller than vector_length * num_comp" );
+    out.data = static_cast<char*>( f.data );
+    out.vl = f.vector_length;
+    out.stride = f.soa_stride;
+    out.ncomp = f.num_comp;
+    if ( f.vector_length >= f.num_particle )
+    {
+        out.shift = 62;
+        out.mask = ( int64_t( 1 ) << 62 ) - 1;
+    }
+    else if ( ( f.vector_length & ( f.vector_length - 1 ) ) == 0 )
+    {
+        int sh = 0;
+        while ( ( int64_t( 1 ) << sh ) < f.vector_length )
+            ++sh;
+        out.shift = sh;
+        out.mask = f.vector_length - 1;
+    }
+    else
+    {
+        out.shift = -1;
+        out.mask = 0;
+    }
+    return CABANA_AMD_OK;
+}
+
+cabana_amd_status make_gfield( cabana_amd_context* ctx, const MeshDev& m,
+                               const cabana_amd_grid_field& g, int dofs,
+                               int dtype, bool allow_null, const char* what,
+                               GFieldDev& out )
+{
+    std::memset( &out, 0, sizeof( out ) );
+    if ( !g.data )
+    {
+        if ( allow_null )
+            return CABANA_AMD_OK;
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     std::string( what ) + ": data is NULL" );
+    }
+    for ( int d = 0; d < 3; ++d )
+        if ( g.extent[d] != m.num_node[d] )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         std::string( what ) +
+                             ": extent does not match the ghosted node space "
+                             "(num_cell + 2 * halo + 1)" );
+    if ( dofs > 0 && g.extent[3] != dofs )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     std::string( what ) + ": expected " +
+                         std::to_string( dofs ) + " dofs, got " +
+                         std::to_string( g.extent[3] ) );
+    if ( g.extent[3] < 1 )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     std::string( what ) + ": no dofs" );
+    if ( dtype >= 0 && g.dtype != dtype )
+        return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED,
+                     std::string( what ) + ": unsupported dtype" );
+    for ( int d = 0; d < 4; ++d )
+        if ( g.stride[d] < 1 && g.extent[d] > 1 )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         std::string( what ) + ": non-positive stride" );
+    out.data = static_cast<char*>( g.data );
+    for ( int d = 0; d < 4; ++d )
+    {
+        out.s[d] = g.stride[d];
+        out.ext[d] = g.extent[d];
+    }
+    return CABANA_AMD_OK;
+}
+
+bool same_grid( const GFieldDev& a, const GFieldDev& b )
+{
+    return a.data == b.data && a.s[0] == b.s[0] && a.s[1] == b.s[1] &&
+           a.s[2] == b.s[2] && a.s[3] == b.s[3];
+}
+
+cabana_amd_status check_order( cabana_amd_context* ctx, int order )
+{
+    if ( order < 1 || order > 3 )
+        return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED,
+                     "spline order must be 1, 2 or 3" );
+    return CABANA_AMD_OK;
+}
+
+int resolve_width( const MeshDev& m, int w ) { return w < 0 ? m.halo : w; }
+
+cabana_amd_status check_width( cabana_amd_context* ctx, const MeshDev& m,
+                               int w )
+{
+    if ( w > m.halo )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "Requested halo width larger than local grid halo" );
+    return CABANA_AMD_OK;
+}
+
+// Particle precision from the first term: FLOAT64 plain or FLOAT32 mixed.
+cabana_amd_status precision_of( cabana_amd_context* ctx, int dtype,
+                                bool& fp64 )
+{
+    if ( dtype == CABANA_AMD_FLOAT64 )
+        fp64 = true;
+    else if ( dtype == CABANA_AMD_FLOAT32 )
+        fp64 = false;
+    else
+        return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED,
+                     "values and grids must be FLOAT64 or FLOAT32" );
+    return CABANA_AMD_OK;
+}
+
+hipError_t run_gather( cabana_amd_context* ctx, int order, bool fp64,
+                       int plan, const GatherArgs& a, hipStream_t s )
+{
+    switch ( order )
+    {
+    case 1:
+        return launch_gather_o1( *ctx, fp64, plan, a, s );
+    case 2:
+        return launch_gather_o2( *ctx, fp64, plan, a, s );
+    default:
+        return launch_gather_o3( *ctx, fp64, plan, a, s );
+    }
+}
+
+hipError_t run_scatter_atomic( cabana_amd_context* ctx, int order, bool fp64,
+                               int plan, const ScatterArgs& a, hipStream_t s )
+{
+    switch ( order )
+    {
+    case 1:
+        return launch_scatter_atomic_o1( *ctx, fp64, plan, a, s );
+    case 2:
+        return launch_scatter_atomic_o2( *ctx, fp64, plan, a, s );
+    default:
+        return launch_scatter_atomic_o3( *ctx, fp64, plan, a, s );
+    }
+}
+
+hipError_t run_scatter_tiled( cabana_amd_context* ctx, int order, bool fp64,
+                              int plan, const ScatterArgs& a,
+                              const TileArgs& t, hipStream_t s )
+{
+    switch ( order )
+    {
+    case 1:
+        return launch_scatter_tiled_o1( *ctx, fp64, plan, a, t, s );
+    case 2:
+        return launch_scatter_tiled_o2( *ctx, fp64, plan, a, t, s );
+    default:
+        return launch_scatter_tiled_o3( *ctx, fp64, plan, a, t, s );
+    }
+}
+
+// Builds tile parameters for cell-sorted particles; returns false when the
+// binning does not describe the mesh's own cells.
+bool make_tiles( const MeshDev& m, int order, bool fp64, int nslot_dofs,
+                 const cabana_amd_binning* bins, const GFieldDev& g0,
+                 TileArgs& t )
+{
+    if ( !bins || !bins->counts || !bins->offsets )
+        return false;
+    for ( int d = 0; d < 3; ++d )
+        if ( bins->num_bin[d] != m.num_cell[d] )
+            return false;
+    const int extra = ( order == 1 ) ? 1 : 3;
+    const size_t real = fp64 ? 8 : 4;
+    const size_t budget = 80 * 1024; // two workgroups per CU
+    int T = 8;
+    while ( T > 2 )
+    {
+        const size_t w = T + extra;
+        if ( w * w * w * nslot_dofs * real + T * T * 12 + 32 <= budget )
+            break;
+        T -= 2;
+    }
+    std::memset( &t, 0, sizeof( t ) );
+    t.cell_offsets = bins->offsets;
+    t.cell_counts = bins->counts;
+    int ntiles = 1;
+    for ( int d = 0; d < 3; ++d )
+    {
+        t.tile[d] = T < m.num_cell[d] ? T : m.num_cell[d];
+        t.ntile[d] = ( m.num_cell[d] + t.tile[d] - 1 ) / t.tile[d];
+        t.win[d] = t.tile[d] + extra;
+        ntiles *= t.ntile[d];
+    }
+    t.tiles_per_xcd = ( ntiles + 7 ) / 8;
+    t.max_runs = t.tile[0] * t.tile[1];
+    t.bin_begin = bins->begin;
+    // Flush order: largest grid stride first.
+    int perm[3] = { 0, 1, 2 };
+    for ( int a = 0; a < 3; ++a )
+        for ( int b = a + 1; b < 3; ++b )
+            if ( g0.s[perm[b]] > g0.s[perm[a]] )
+            {
+                int tmp = perm[a];
+                perm[a] = perm[b];
+                perm[b] = tmp;
+            }
+    for ( int d = 0; d < 3; ++d )
+        t.flush_perm[d] = perm[d];
+    return true;
+}
+
+// Runs a scatter pass over all particles: tiled over [bins.begin, bins.end)
+// when the binning matches the mesh, atomic elsewhere.
+cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
+                                int plan, ScatterArgs a, int64_t n,
+                                const cabana_amd_binning* bins,
+                                hipStream_t s )
+{
+    int dofs = 0;
+    if ( plan == PLAN_MPM_G2P )
+        dofs = 3;
+    else
+        for ( int i = 0; i < a.nterms; ++i )
+            dofs += form_gdof( a.t[i].form );
+    TileArgs t;
+    GFieldDev g0 = a.t[0].grid.data ? a.t[0].grid : a.t[0].gx;
+    const bool tiled =
+        bins && bins->begin >= 0 && bins->begin <= bins->end &&
+        bins->end <= n && make_tiles( a.mesh, order, fp64, dofs, bins, g0, t );
+    if ( tiled )
+    {
+        if ( bins->begin > 0 )
+        {
+            a.begin = 0;
+            a.end = bins->begin;
+            CA_HIP( ctx, run_scatter_atomic( ctx, order, fp64, plan, a, s ),
+                    "scatter (atomic)" );
+        }
+        if ( bins->end > bins->begin )
+        {
+            a.begin = bins->begin;
+            a.end = bins->end;
+            CA_HIP( ctx, run_scatter_tiled( ctx, order, fp64, plan, a, t, s ),
+                    "scatter (tiled)" );
+        }
+        if ( bins->end < n )
+        {
+            a.begin = bins->end;
+            a.end = n;
+            CA_HIP( ctx, run_scatter_atomic( ctx, order, fp64, plan, a, s ),
+                    "scatter (atomic)" );
+        }
+    }
+    else
+    {
+        a.begin = 0;
+        a.end = n;
+        CA_HIP( ctx, run_scatter_atomic( ctx, order, fp64, plan, a, s ),
+                "scatter (atomic)" );
+    }
+    return CABANA_AMD_OK;
+}
+
+cabana_amd_status validate_bins( cabana_amd_context* ctx,
+                                 const cabana_amd_binning* bins, int64_t n )
+{
+    if ( !bins )
+        return CABANA_AMD_OK;
+    if ( bins->begin < 0 || bins->end < bins->begin || bins->end > n )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "binning range outside the particle range" );
+    return CABANA_AMD_OK;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+extern "C" int32_t cabana_amd_abi_version( void ) { return CABANA_AMD_ABI_VERSION; }
+
+extern "C" cabana_amd_status cabana_amd_context_create( int32_t device,
+                                                        cabana_amd_context** out )
+{
+    if ( !out )
+        return fail( nullptr, CABANA_AMD_ERROR_INVALID_ARGUMENT, "out is NULL" );
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount( &count );
+    if ( e != hipSuccess || count < 1 )
+        return fail( nullptr, CABANA_AMD_ERROR_DEVICE, "no HIP device available" );
+    if ( device < 0 || device >= count )
+        return fail( nullptr, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "device index out of range" );
+    e = hipSetDevice( device );
+    if ( e != hipSuccess )
+        return hip_fail( nullptr, e, "hipSetDevice" );
+    auto* ctx = new cabana_amd_context();
+    ctx->device = device;
+    e = hipMalloc( &ctx->skipped, 2 * sizeof( unsigned long long ) );
+    if ( e != hipSuccess )
+    {
+        delete ctx;
+        return hip_fail( nullptr, e, "hipMalloc" );
+    }
+    ctx->bad = ctx->skipped + 1;
+    e = hipMemset( ctx->skipped, 0, 2 * sizeof( unsigned long long ) );
+    if ( e != hipSuccess )
+    {
+        hipFree( ctx->skipped );
+        delete ctx;
+        return hip_fail( nullptr, e, "hipMemset" );
+    }
+    *out = ctx;
+    return CABANA_AMD_OK;
+}
+
+extern "C" cabana_amd_status cabana_amd_context_destroy( cabana_amd_context* ctx )
+{
+    if ( !ctx )
+        return CABANA_AMD_OK;
+    hipSetDevice( ctx->device );
+    hipDeviceSynchronize();
+    ctx->drain();
+    for ( auto e : ctx->pool )
+        hipEventDestroy( e );
+    for ( int i = 0; i < 4; ++i )
+        if ( ctx->scratch[i] )
+            hipFree( ctx->scratch[i] );
+    if ( ctx->skipped )
+        hipFree( ctx->skipped );
+    delete ctx;
+    return CABANA_AMD_OK;
+}
+
+extern "C" const char* cabana_amd_last_error( const cabana_amd_context* ctx )
+{
+    if ( ctx )
+        return ctx->error.c_str();
+    return g_thread_error.c_str();
+}
+
+extern "C" cabana_amd_status cabana_amd_check_domain( cabana_amd_context* ctx,
+                                                      cabana_amd_stream stream,
+                                                      int64_t* num_skipped )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    CA_HIP( ctx, hipStreamSynchronize( s ), "hipStreamSynchronize" );
+    unsigned long long v = 0;
+    CA_HIP( ctx,
+            hipMemcpy( &v, ctx->skipped, sizeof( v ), hipMemcpyDeviceToHost ),
+            "hipMemcpy" );
+    CA_HIP( ctx, hipMemset( ctx->skipped, 0, sizeof( v ) ), "hipMemset" );
+    if ( num_skipped )
+        *num_skipped = (int64_t)v;
+    if ( v )
+        return fail( ctx, CABANA_AMD_ERROR_OUT_OF_DOMAIN,
+                     std::to_string( v ) +
+                         " particles had stencils outside the ghosted grid" );
+    return CABANA_AMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// P2G / G2P
+// ---------------------------------------------------------------------------
+
+extern "C" cabana_amd_status cabana_amd_p2g(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
+    const cabana_amd_particle_field* points, int32_t num_terms,
+    const cabana_amd_p2g_term* terms, int32_t halo_width,
+    const cabana_amd_binning* bins, cabana_amd_stream stream )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    MeshDev m;
+    CA_CHECK( make_mesh( ctx, mesh, m ) );
+    CA_CHECK( check_order( ctx, order ) );
+    if ( !points || num_terms < 1 || num_terms > MAX_TERMS || !terms )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "need points and 1..3 terms" );
+    const int w = resolve_width( m, halo_width );
+    CA_CHECK( check_width( ctx, m, w ) );
+    const int64_t n = points->num_particle;
+    CA_CHECK( validate_bins( ctx, bins, n ) );
+    bool fp64 = true;
+    CA_CHECK( precision_of( ctx, terms[0].values.dtype, fp64 ) );
+    const int rdt = fp64 ? CABANA_AMD_FLOAT64 : CABANA_AMD_FLOAT32;
+
+    ScatterArgs a;
+    std::memset( &a, 0, sizeof( a ) );
+    a.mesh = m;
+    a.skipped = ctx->skipped;
+    CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, n, false,
+                           "points", a.points ) );
+    for ( int i = 0; i < num_terms; ++i )
+    {
+        const auto& t = terms[i];
+        if ( t.op < 0 || t.op > 4 )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "unknown P2G operator" );
+        const int form = p2g_form( t.op );
+        a.t[i].form = form;
+        a.t[i].mult = t.multiplier;
+        CA_CHECK( make_pfield( ctx, t.values, form_pcomp( form ), rdt, n,
+                               false, "values", a.t[i].pin ) );
+        CA_CHECK( make_gfield( ctx, m, t.grid, form_gdof( form ), rdt, false,
+                               "grid", a.t[i].grid ) );
+    }
+    for ( int i = 0; i < num_terms; ++i )
+        for ( int j = 0; j < i; ++j )
+            if ( a.t[i].grid.data == a.t[j].grid.data &&
+                 !same_grid( a.t[i].grid, a.t[j].grid ) )
+                return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                             "terms share grid memory with different layouts" );
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    if ( n > 0 )
+    {
+        const bool mpm = num_terms == 3 && a.t[0].form == FORM_SV &&
+                         a.t[1].form == FORM_VV && a.t[2].form == FORM_TD;
+        if ( num_terms == 1 || mpm )
+        {
+            a.nterms = num_terms;
+            CA_CHECK( scatter_pass( ctx, order, fp64,
+                                    mpm ? PLAN_MPM_P2G : PLAN_SINGLE, a, n,
+                                    bins, s ) );
+        }
+        else
+        {
+            for ( int i = 0; i < num_terms; ++i )
+            {
+                ScatterArgs b = a;
+                b.nterms = 1;
+                b.t[0] = a.t[i];
+                CA_CHECK( scatter_pass( ctx, order, fp64, PLAN_SINGLE, b, n,
+                                        bins, s ) );
+            }
+        }
+    }
+    // Fold every distinct grid once.
+    for ( int i = 0; i < num_terms; ++i )
+    {
+        bool seen = false;
+        for ( int j = 0; j < i; ++j )
+            seen = seen || a.t[j].grid.data == a.t[i].grid.data;
+        if ( !seen )
+            CA_HIP( ctx,
+                    launch_halo_scatter_sum( *ctx, fp64, m, a.t[i].grid, w, s ),
+                    "halo scatter" );
+    }
+    return CABANA_AMD_OK;
+}
+
+extern "C" cabana_amd_status cabana_amd_g2p(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
+    const cabana_amd_particle_field* points, int32_t num_terms,
+    const cabana_amd_g2p_term* terms, int32_t halo_width,
+    const cabana_amd_binning* bins, cabana_amd_stream stream )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    MeshDev m;
+    CA_CHECK( make_mesh( ctx, mesh, m ) );
+    CA_CHECK( check_order( ctx, order ) );
+    if ( !points || num_terms < 1 || num_terms > MAX_TERMS || !terms )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "need points and 1..3 terms" );
+    const int w = resolve_width( m, halo_width );
+    CA_CHECK( check_width( ctx, m, w ) );
+    const int64_t n = points->num_particle;
+    CA_CHECK( validate_bins( ctx, bins, n ) );
+    bool fp64 = true;
+    CA_CHECK( precision_of( ctx, terms[0].values.dtype, fp64 ) );
+    const int rdt = fp64 ? CABANA_AMD_FLOAT64 : CABANA_AMD_FLOAT32;
+
+    GatherArgs a;
+    std::memset( &a, 0, sizeof( a ) );
+    a.mesh = m;
+    a.skipped = ctx->skipped;
+    CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, n, false,
+                           "points", a.points ) );
+    for ( int i = 0; i < num_terms; ++i )
+    {
+        const auto& t = terms[i];
+        if ( t.op < 0 || t.op > 4 )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "unknown G2P operator" );
+        const int form = g2p_form( t.op );
+        a.t[i].form = form;
+        a.t[i].mult = t.multiplier;
+        CA_CHECK( make_pfield( ctx, t.values, form_pcomp( form ), rdt, n,
+                               false, "values", a.t[i].pout ) );
+        CA_CHECK( make_gfield( ctx, m, t.grid, form_gdof( form ), rdt, false,
+                               "grid", a.t[i].grid ) );
+    }
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    for ( int i = 0; i < num_terms; ++i )
+    {
+        bool seen = false;
+        for ( int j = 0; j < i; ++j )
+            seen = seen || a.t[j].grid.data == a.t[i].grid.data;
+        if ( !seen )
+            CA_HIP( ctx, launch_halo_gather( *ctx, fp64, m, a.t[i].grid, w, s ),
+                    "halo gather" );
+    }
+    if ( n <= 0 )
+        return CABANA_AMD_OK;
+    a.begin = 0;
+    a.end = n;
+    const bool pair = num_terms == 2 && a.t[0].form == FORM_VV &&
+                      a.t[1].form == FORM_TDT &&
+                      same_grid( a.t[0].grid, a.t[1].grid );
+    if ( num_terms == 1 || pair )
+    {
+        a.nterms = num_terms;
+        CA_HIP( ctx,
+                run_gather( ctx, order, fp64, pair ? PLAN_MPM_G2P : PLAN_SINGLE,
+                            a, s ),
+                "gather" );
+    }
+    else
+    {
+        for ( int i = 0; i < num_terms; ++i )
+        {
+            GatherArgs b = a;
+            b.nterms = 1;
+            b.t[0] = a.t[i];
+            CA_HIP( ctx, run_gather( ctx, order, fp64, PLAN_SINGLE, b, s ),
+                    "gather" );
+        }
+    }
+    return CABANA_AMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoints
+// ---------------------------------------------------------------------------
+
+extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
+    const cabana_amd_particle_field* points, int32_t num_terms,
+    const cabana_amd_p2g_adjoint_term* terms, int32_t halo_width,
+    const cabana_amd_particle_field* points_bar,
+    const cabana_amd_binning* bins, cabana_amd_stream stream )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    MeshDev m;
+    CA_CHECK( make_mesh( ctx, mesh, m ) );
+    CA_CHECK( check_order( ctx, order ) );
+    if ( !points || num_terms < 1 || num_terms > MAX_TERMS || !terms )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "need points and 1..3 terms" );
+    const int w = resolve_width( m, halo_width );
+    CA_CHECK( check_width( ctx, m, w ) );
+    const int64_t n = points->num_particle;
+    CA_CHECK( validate_bins( ctx, bins, n ) );
+    bool fp64 = true;
+    CA_CHECK( precision_of( ctx, terms[0].grid_bar.dtype, fp64 ) );
+    const int rdt = fp64 ? CABANA_AMD_FLOAT64 : CABANA_AMD_FLOAT32;
+    const bool xbar = points_bar && points_bar->data;
+
+    GatherArgs a;
+    std::memset( &a, 0, sizeof( a ) );
+    a.mesh = m;
+    a.skipped = ctx->skipped;
+    CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, n, false,
+                           "points", a.points ) );
+    if ( xbar )
+        CA_CHECK( make_pfield( ctx, *points_bar, 3, CABANA_AMD_FLOAT64, n,
+                               false, "points_bar", a.xbar ) );
+    a.has_xbar = xbar ? 1 : 0;
+    for ( int i = 0; i < num_terms; ++i )
+    {
+        const auto& t = terms[i];
+        if ( t.op < 0 || t.op > 4 )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "unknown P2G operator" );
+        const int form = p2g_form( t.op );
+        a.t[i].form = form;
+        a.t[i].mult = t.multiplier;
+        CA_CHECK( make_pfield( ctx, t.values_bar, form_pcomp( form ), rdt, n,
+                               true, "values_bar", a.t[i].pout ) );
+        CA_CHECK( make_pfield( ctx, t.values, form_pcomp( form ), rdt, n,
+                               !xbar, "values", a.t[i].px ) );
+        CA_CHECK( make_gfield( ctx, m, t.grid_bar, form_gdof( form ), rdt,
+                               false, "grid_bar", a.t[i].grid ) );
+    }
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    for ( int i = 0; i < num_terms; ++i )
+    {
+        bool seen = false;
+        for ( int j = 0; j < i; ++j )
+            seen = seen || a.t[j].grid.data == a.t[i].grid.data;
+        if ( !seen )
+            CA_HIP( ctx,
+                    launch_halo_adjoint_fill( *ctx, fp64, m, a.t[i].grid, w, s ),
+                    "halo adjoint fill" );
+    }
+    if ( n <= 0 )
+        return CABANA_AMD_OK;
+    a.begin = 0;
+    a.end = n;
+    const bool mpm = num_terms == 3 && xbar && a.t[0].form == FORM_SV &&
+                     a.t[1].form == FORM_VV && a.t[2].form == FORM_TD;
+    if ( num_terms == 1 || mpm )
+    {
+        a.nterms = num_terms;
+        CA_HIP( ctx,
+                run_gather( ctx, order, fp64, mpm ? PLAN_MPM_P2G : PLAN_SINGLE,
+                            a, s ),
+                "p2g adjoint gather" );
+    }
+    else
+    {
+        for ( int i = 0; i < num_terms; ++i )
+        {
+            GatherArgs b = a;
+            b.nterms = 1;
+            b.t[0] = a.t[i];
+            CA_HIP( ctx, run_gather( ctx, order, fp64, PLAN_SINGLE, b, s ),
+                    "p2g adjoint gather" );
+        }
+    }
+    (void)bins;
+    return CABANA_AMD_OK;
+}
+
+extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
+    const cabana_amd_particle_field* points, int32_t num_terms,
+    const cabana_amd_g2p_adjoint_term* terms, int32_t halo_width,
+    const cabana_amd_particle_field* points_bar,
+    const cabana_amd_binning* bins, cabana_amd_stream stream )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    MeshDev m;
+    CA_CHECK( make_mesh( ctx, mesh, m ) );
+    CA_CHECK( check_order( ctx, order ) );
+    if ( !points || num_terms < 1 || num_terms > MAX_TERMS || !terms )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "need points and 1..3 terms" );
+    const int w = resolve_width( m, halo_width );
+    CA_CHECK( check_width( ctx, m, w ) );
+    const int64_t n = points->num_particle;
+    CA_CHECK( validate_bins( ctx, bins, n ) );
+    bool fp64 = true;
+    CA_CHECK( precision_of( ctx, terms[0].grid.dtype, fp64 ) );
+    const int rdt = fp64 ? CABANA_AMD_FLOAT64 : CABANA_AMD_FLOAT32;
+    const bool xbar = points_bar && points_bar->data;
+
+    ScatterArgs a;
+    std::memset( &a, 0, sizeof( a ) );
+    a.mesh = m;
+    a.skipped = ctx->skipped;
+    CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, n, false,
+                           "points", a.points ) );
+    if ( xbar )
+        CA_CHECK( make_pfield( ctx, *points_bar, 3, CABANA_AMD_FLOAT64, n,
+                               false, "points_bar", a.xbar ) );
+    a.has_xbar = xbar ? 1 : 0;
+    for ( int i = 0; i < num_terms; ++i )
+    {
+        const auto& t = terms[i];
+        if ( t.op < 0 || t.op > 4 )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "unknown G2P operator" );
+        const int form = g2p_form( t.op );
+        a.t[i].form = form;
+        a.t[i].mult = t.multiplier;
+        CA_CHECK( make_pfield( ctx, t.values_bar, form_pcomp( form ), rdt, n,
+                               false, "values_bar", a.t[i].pin ) );
+        CA_CHECK( make_gfield( ctx, m, t.grid_bar, form_gdof( form ), rdt,
+                               true, "grid_bar", a.t[i].grid ) );
+        CA_CHECK( make_gfield( ctx, m, t.grid, form_gdof( form ), rdt, !xbar,
+                               "grid", a.t[i].gx ) );
+        if ( !a.t[i].grid.data && !xbar )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "term has neither grid_bar nor points_bar" );
+    }
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    // Forward ghost fill of the grids read for the position cotangent.
+    if ( xbar )
+        for ( int i = 0; i < num_terms; ++i )
+        {
+            bool seen = false;
+            for ( int j = 0; j < i; ++j )
+                seen = seen || a.t[j].gx.data == a.t[i].gx.data;
+            if ( !seen )
+                CA_HIP( ctx,
+                        launch_halo_gather( *ctx, fp64, m, a.t[i].gx, w, s ),
+                        "halo gather" );
+        }
+    // Cotangent grids: clear ghosts, scatter, fold.
+    for ( int i = 0; i < num_terms; ++i )
+    {
+        bool seen = false;
+        for ( int j = 0; j < i; ++j )
+            seen = seen || a.t[j].grid.data == a.t[i].grid.data;
+        if ( !seen && a.t[i].grid.data )
+            CA_HIP( ctx, launch_zero_ghosts( *ctx, fp64, m, a.t[i].grid, s ),
+                    "zero ghosts" );
+    }
+    if ( n > 0 )
+    {
+        const bool pair = num_terms == 2 && xbar && a.t[0].form == FORM_VV &&
+                          a.t[1].form == FORM_TDT && a.t[0].grid.data &&
+                          same_grid( a.t[0].grid, a.t[1].grid ) &&
+                          same_grid( a.t[0].gx, a.t[1].gx );
+        if ( num_terms == 1 || pair )
+        {
+            a.nterms = num_terms;
+            CA_CHECK( scatter_pass( ctx, order, fp64,
+                                    pair ? PLAN_MPM_G2P : PLAN_SINGLE, a, n,
+                                    bins, s ) );
+        }
+        else
+        {
+            for ( int i = 0; i < num_terms; ++i )
+            {
+                ScatterArgs b = a;
+                b.nterms = 1;
+                b.t[0] = a.t[i];
+                CA_CHECK( scatter_pass( ctx, order, fp64, PLAN_SINGLE, b, n,
+                                        bins, s ) );
+            }
+        }
+    }
+    for ( int i = 0; i < num_terms; ++i )
+    {
+        bool seen = false;
+        for ( int j = 0; j < i; ++j )
+            seen = seen || a.t[j].grid.data == a.t[i].grid.data;
+        if ( !seen && a.t[i].grid.data )
+            CA_HIP( ctx,
+                    launch_halo_scatter_sum( *ctx, fp64, m, a.t[i].grid, w, s ),
+                    "halo scatter" );
+    }
+    return CABANA_AMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Halo
+// ---------------------------------------------------------------------------
+
+namespace
+{
+cabana_amd_status halo_common( cabana_amd_context* ctx,
+                               const cabana_amd_mesh* mesh,
+                               const cabana_amd_grid_field* grid,
+                               int32_t halo_width, MeshDev& m, GFieldDev& g,
+                               int& w, bool& fp64 )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    CA_CHECK( make_mesh( ctx, mesh, m ) );
+    if ( !grid )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "grid is NULL" );
+    CA_CHECK( precision_of( ctx, grid->dtype, fp64 ) );
+    CA_CHECK( make_gfield( ctx, m, *grid, -1, grid->dtype, false, "grid", g ) );
+    w = resolve_width( m, halo_width );
+    return check_width( ctx, m, w );
+}
+} // namespace
+
+extern "C" cabana_amd_status cabana_amd_halo_gather(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+    const cabana_amd_grid_field* grid, int32_t halo_width,
+    cabana_amd_stream stream )
+{
+    MeshDev m;
+    GFieldDev g;
+    int w;
+    bool fp64;
+    CA_CHECK( halo_common( ctx, mesh, grid, halo_width, m, g, w, fp64 ) );
+    CA_HIP( ctx,
+            launch_halo_gather( *ctx, fp64, m, g, w,
+                                static_cast<hipStream_t>( stream ) ),
+            "halo gather" );
+    return CABANA_AMD_OK;
+}
+
+extern "C" cabana_amd_status cabana_amd_halo_scatter_sum(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+    const cabana_amd_grid_field* grid, int32_t halo_width,
+    cabana_amd_stream stream )
+{
+    MeshDev m;
+    GFieldDev g;
+    int w;
+    bool fp64;
+    CA_CHECK( halo_common( ctx, mesh, grid, halo_width, m, g, w, fp64 ) );
+    CA_HIP( ctx,
+            launch_halo_scatter_sum( *ctx, fp64, m, g, w,
+                                     static_cast<hipStream_t>( stream ) ),
+            "halo scatter" );
+    return CABANA_AMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Binning
+// ---------------------------------------------------------------------------
+
+namespace
+{
+// CartesianGrid::init (core/src/impl/Cabana_CartesianGrid.hpp:61-73,279-283).
+bool cart_init( const double delta[3], const double min[3],
+                const double max[3], CartGridDev& g, int64_t& ncell )
+{
+    ncell = 1;
+    for ( int d = 0; d < 3; ++d )
+    {
+        if ( !( delta[d] > 0.0 ) || !( max[d] > min[d] ) )
+            return false;
+        const double nx = std::floor( ( max[d] - min[d] ) * ( 1.0 / delta[d] ) );
+        if ( nx < 1.0 || nx > 2.0e9 )
+            return false;
+        g.nx[d] = (int32_t)nx;
+        const double dx = ( max[d] - min[d] ) / g.nx[d];
+        g.rdx[d] = 1.0 / dx;
+        g.min[d] = min[d];
+        ncell *= g.nx[d];
+    }
+    return ncell <= ( int64_t( 1 ) << 31 ) - 1;
+}
+} // namespace
+
+extern "C" cabana_amd_status cabana_amd_linked_cell_num_bins(
+    const double grid_delta[3], const double grid_min[3],
+    const double grid_max[3], int32_t num_bin[3] )
+{
+    if ( !grid_delta || !grid_min || !grid_max || !num_bin )
+        return fail( nullptr, CABANA_AMD_ERROR_INVALID_ARGUMENT, "NULL argument" );
+    CartGridDev g;
+    int64_t ncell;
+    if ( !cart_init( grid_delta, grid_min, grid_max, g, ncell ) )
+        return fail( nullptr, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "invalid Cartesian grid" );
+    for ( int d = 0; d < 3; ++d )
+        num_bin[d] = g.nx[d];
+    return CABANA_AMD_OK;
+}
+
+extern "C" cabana_amd_status cabana_amd_linked_cell_build(
+    cabana_amd_context* ctx, const cabana_amd_particle_field* positions,
+    int64_t begin, int64_t end, const double grid_delta[3],
+    const double grid_min[3], const double grid_max[3], int32_t* counts,
+    int64_t* offsets, int64_t* permutes, cabana_amd_stream stream )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    if ( !positions || !grid_delta || !grid_min || !grid_max || !counts ||
+         !offsets || ( !permutes && end > begin ) )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "NULL argument" );
+    if ( begin < 0 || end < begin || end > positions->num_particle )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "begin/end outside the particle range" );
+    PFieldDev pos;
+    CA_CHECK( make_pfield( ctx, *positions, 3, CABANA_AMD_FLOAT64, end, false,
+                           "positions", pos ) );
+    CartGridDev g;
+    int64_t ncell;
+    if ( !cart_init( grid_delta, grid_min, grid_max, g, ncell ) )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "invalid Cartesian grid" );
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    const int64_t np = end - begin;
+    const int64_t nblock = ( ncell + 1023 ) / 1024;
+    CA_HIP( ctx, ctx->ensure( SLOT_CELL_OF, sizeof( int32_t ) * ( np + 1 ) ),
+            "scratch" );
+    CA_HIP( ctx, ctx->ensure( SLOT_CURSOR, sizeof( int64_t ) * ( ncell + 1 ) ),
+            "scratch" );
+    CA_HIP( ctx, ctx->ensure( SLOT_SCAN, sizeof( int64_t ) * ( nblock + 1 ) ),
+            "scratch" );
+    int32_t* cell_of = static_cast<int32_t*>( ctx->scratch[SLOT_CELL_OF] );
+    int64_t* cursor = static_cast<int64_t*>( ctx->scratch[SLOT_CURSOR] );
+    int64_t* sums = static_cast<int64_t*>( ctx->scratch[SLOT_SCAN] );
+    CA_HIP( ctx,
+            launch_fill_i32( *ctx, reinterpret_cast<int32_t*>( ctx->bad ), 2, 0,
+                             s ),
+            "fill" );
+    CA_HIP( ctx, launch_fill_i32( *ctx, counts, ncell, 0, s ), "fill" );
+    CA_HIP( ctx,
+            launch_lcl_count( *ctx, pos, begin, end, g, counts, cell_of,
+                              ctx->bad, s ),
+            "count" );
+    CA_HIP( ctx,
+            launch_exclusive_scan( *ctx, counts, offsets, ncell, sums,
+                                   nblock + 1, s ),
+            "scan" );
+    CA_HIP( ctx, launch_copy_i64( *ctx, offsets, cursor, ncell, s ), "copy" );
+    CA_HIP( ctx,
+            launch_lcl_place( *ctx, begin, end, cell_of, cursor, permutes, s ),
+            "place" );
+    CA_HIP( ctx, launch_lcl_order_bins( *ctx, offsets, counts, ncell, permutes, s ),
+            "order bins" );
+    // The reference fences after each phase; one wait here reports
+    // particles outside the grid.
+    unsigned long long bad = 0;
+    CA_HIP( ctx,
+            hipMemcpyAsync( &bad, ctx->bad, sizeof( bad ),
+                            hipMemcpyDeviceToHost, s ),
+            "hipMemcpyAsync" );
+    CA_HIP( ctx, hipStreamSynchronize( s ), "hipStreamSynchronize" );
+    if ( bad )
+        return fail( ctx, CABANA_AMD_ERROR_OUT_OF_DOMAIN,
+                     std::to_string( bad ) +
+                         " particles lie outside the binning grid" );
+    return CABANA_AMD_OK;
+}
+
+extern "C" cabana_amd_status cabana_amd_permute(
+    cabana_amd_context* ctx, const int64_t* permutes, int64_t begin,
+    int64_t end, const cabana_amd_particle_field* field,
+    cabana_amd_stream stream )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    if ( !field || ( !permutes && end > begin ) )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "NULL argument" );
+    if ( begin < 0 || end < begin || end > field->num_particle )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "begin/end outside the particle range" );
+    const int es = dtype_size( field->dtype );
+    if ( !es )
+        return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED, "unknown dtype" );
+    PFieldDev f;
+    CA_CHECK( make_pfield( ctx, *field, -1, -1, end, false, "field", f ) );
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    const int64_t np = end - begin;
+    CA_HIP( ctx,
+            ctx->ensure( SLOT_PERMUTE, (size_t)es * f.ncomp * ( np + 1 ) ),
+            "scratch" );
+    void* scratch = ctx->scratch[SLOT_PERMUTE];
+    CA_HIP( ctx,
+            launch_permute_gather( *ctx, f, es, permutes, begin, end, scratch, s ),
+            "permute gather" );
+    CA_HIP( ctx, launch_permute_copy_back( *ctx, f, es, begin, end, scratch, s ),
+            "permute copy back" );
+    return CABANA_AMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Profiling
+// ---------------------------------------------------------------------------
+
+extern "C" cabana_amd_status cabana_amd_profile_enable( cabana_amd_context* ctx,
+                                                        int32_t enable )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    ctx->profiling = enable != 0;
+    return CABANA_AMD_OK;
+}
+
+extern "C" cabana_amd_status cabana_amd_profile_reset( cabana_amd_context* ctx )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    CA_HIP( ctx, ctx->drain(), "event synchronize" );
+    ctx->stats.clear();
+    ctx->order.clear();
+    return CABANA_AMD_OK;
+}
+
+extern "C" cabana_amd_status cabana_amd_profile_count( cabana_amd_context* ctx,
+                                                       int32_t* count )
+{
+    if ( !ctx || !count )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "NULL argument" );
+    CA_HIP( ctx, ctx->drain(), "event synchronize" );
+    *count = (int32_t)ctx->order.size();
+    return CABANA_AMD_OK;
+}
+
+extern "C" cabana_amd_status cabana_amd_profile_entry(
+    cabana_amd_context* ctx, int32_t index, char* name, int32_t name_capacity,
+    int64_t* launches, double* total_ms )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    if ( index < 0 || index >= (int32_t)ctx->order.size() )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "profile index out of range" );
+    const std::string& nm = ctx->order[index];
+    const Stat& st = ctx->stats[nm];
+    if ( name && name_capacity > 0 )
+    {
+        std::snprintf( name, (size_t)name_capacity, "%s", nm.c_str() );
+    }
+    if ( launches )
+        *launches = st.launches;
+    if ( total_ms )
+        *total_ms = st.ms;
+    return CABANA_AMD_OK;
+}

@@ -1,0 +1,583 @@
+// Grid-side kernels: the single-rank periodic node halo (fill and fold) and
+// the cell binning (LinkedCellList build + permute) used to order particles.
+#include "ca_internal.hpp"
+
+namespace cabana_amd
+{
+namespace
+{
+
+constexpr int BLOCK = 256;
+
+inline dim3 blocks_for( int64_t n )
+{
+    return dim3( (unsigned)( ( n + BLOCK - 1 ) / BLOCK ) );
+}
+
+// Owned node range of one dimension on one rank
+// (Cabana_Grid_LocalGrid_impl.hpp:430-450): periodic dimensions own N nodes,
+// non-periodic ones also own the closing boundary node.
+__device__ __forceinline__ int owned_hi( const MeshDev& m, int d )
+{
+    return m.halo + m.num_cell[d] + ( m.periodic[d] ? 0 : 1 );
+}
+
+// Classifies node index x of dimension d for a halo of width w.  Returns the
+// owner index (itself when owned), or -1 when the node is a ghost that no
+// message of the 26-direction exchange reaches.  The bands are the shared
+// index spaces of Cabana_Grid_LocalGrid_impl.hpp:470-509: low ghosts
+// [h - w, h) receive from [h + N - w, h + N), high ghosts [h + N, h + N + w + 1)
+// receive from [h, h + w + 1).
+__device__ __forceinline__ int owner_of( const MeshDev& m, int d, int x, int w,
+                                         bool& is_ghost )
+{
+    const int h = m.halo, N = m.num_cell[d];
+    is_ghost = false;
+    if ( x >= h && x < owned_hi( m, d ) )
+        return x;
+    is_ghost = true;
+    if ( !m.periodic[d] )
+        return -1;
+    if ( x >= h - w && x < h )
+        return x + N;
+    if ( x >= h + N && x < h + N + w + 1 )
+        return x - N;
+    return -1;
+}
+
+template <class Real>
+__global__ __launch_bounds__( BLOCK ) void halo_fill_kernel( MeshDev m,
+                                                             GFieldDev g,
+                                                             int w,
+                                                             int zero_other )
+{
+    const int64_t n = (int64_t)g.ext[0] * g.ext[1] * g.ext[2];
+    const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( e >= n )
+        return;
+    const int i = (int)( e / ( (int64_t)g.ext[1] * g.ext[2] ) );
+    const int j = (int)( ( e / g.ext[2] ) % g.ext[1] );
+    const int k = (int)( e % g.ext[2] );
+    bool gi, gj, gk;
+    const int oi = owner_of( m, 0, i, w, gi );
+    const int oj = owner_of( m, 1, j, w, gj );
+    const int ok = owner_of( m, 2, k, w, gk );
+    if ( !gi && !gj && !gk )
+        return; // owned
+    Real* dst = reinterpret_cast<Real*>( g.data ) + i * g.s[0] + j * g.s[1] +
+                k * g.s[2];
+    if ( oi < 0 || oj < 0 || ok < 0 )
+    {
+        if ( zero_other )
+            for ( int d = 0; d < g.ext[3]; ++d )
+                dst[d * g.s[3]] = Real( 0 );
+        return;
+    }
+    const Real* src = reinterpret_cast<const Real*>( g.data ) + oi * g.s[0] +
+                      oj * g.s[1] + ok * g.s[2];
+    for ( int d = 0; d < g.ext[3]; ++d )
+        dst[d * g.s[3]] = src[d * g.s[3]];
+}
+
+// Fold: every owned node adds all of its periodic ghost images.  Each owner
+// gathers its images itself, so the sum needs no atomics and its order is
+// fixed (the reference's MPI Waitany order is not).
+template <class Real>
+__global__ __launch_bounds__( BLOCK ) void halo_fold_kernel( MeshDev m,
+                                                             GFieldDev g,
+                                                             int w )
+{
+    int lo[3], n[3];
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        lo[d] = m.halo;
+        n[d] = owned_hi( m, d ) - m.halo;
+    }
+    const int64_t total = (int64_t)n[0] * n[1] * n[2];
+    const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( e >= total )
+        return;
+    const int x[3] = { lo[0] + (int)( e / ( (int64_t)n[1] * n[2] ) ),
+                       lo[1] + (int)( ( e / n[2] ) % n[1] ),
+                       lo[2] + (int)( e % n[2] ) };
+    int cand[3][3], nc[3];
+    bool any = false;
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        const int h = m.halo, N = m.num_cell[d];
+        cand[d][0] = x[d];
+        nc[d] = 1;
+        if ( m.periodic[d] )
+        {
+            if ( x[d] < h + w + 1 )
+                cand[d][nc[d]++] = x[d] + N;
+            if ( x[d] >= h + N - w )
+                cand[d][nc[d]++] = x[d] - N;
+        }
+        any = any || nc[d] > 1;
+    }
+    if ( !any )
+        return;
+    Real* dst = reinterpret_cast<Real*>( g.data ) + x[0] * g.s[0] +
+                x[1] * g.s[1] + x[2] * g.s[2];
+    for ( int d = 0; d < g.ext[3]; ++d )
+    {
+        Real acc = Real( 0 );
+        for ( int a = 0; a < nc[0]; ++a )
+            for ( int b = 0; b < nc[1]; ++b )
+                for ( int c = 0; c < nc[2]; ++c )
+                {
+                    if ( a == 0 && b == 0 && c == 0 )
+                        continue;
+                    acc += reinterpret_cast<const Real*>(
+                        g.data )[cand[0][a] * g.s[0] + cand[1][b] * g.s[1] +
+                                 cand[2][c] * g.s[2] + d * g.s[3]];
+                }
+        dst[d * g.s[3]] += acc;
+    }
+}
+
+template <class Real>
+__global__ __launch_bounds__( BLOCK ) void zero_ghosts_kernel( MeshDev m,
+                                                               GFieldDev g )
+{
+    const int64_t n = (int64_t)g.ext[0] * g.ext[1] * g.ext[2];
+    const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( e >= n )
+        return;
+    const int x[3] = { (int)( e / ( (int64_t)g.ext[1] * g.ext[2] ) ),
+                       (int)( ( e / g.ext[2] ) % g.ext[1] ),
+                       (int)( e % g.ext[2] ) };
+    bool owned = true;
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+        owned = owned && x[d] >= m.halo && x[d] < owned_hi( m, d );
+    if ( owned )
+        return;
+    Real* dst = reinterpret_cast<Real*>( g.data ) + x[0] * g.s[0] +
+                x[1] * g.s[1] + x[2] * g.s[2];
+    for ( int d = 0; d < g.ext[3]; ++d )
+        dst[d * g.s[3]] = Real( 0 );
+}
+
+// ---------------------------------------------------------------------------
+// Binning
+// ---------------------------------------------------------------------------
+
+// locatePoint + cardinalCellIndex (Cabana_CartesianGrid.hpp:171-182,
+// 226-231, 279-283) with the same FP64 rounding: floor((x - min) * rdx),
+// clamped only when it equals nx.
+__device__ __forceinline__ int64_t locate_cell( const CartGridDev& g,
+                                                const double p[3] )
+{
+#pragma clang fp contract( off )
+    int c[3];
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        const double f = floor( ( p[d] - g.min[d] ) * g.rdx[d] );
+        if ( !( f >= 0.0 && f <= (double)g.nx[d] ) )
+            return -1;
+        int v = (int)f;
+        v = ( v == g.nx[d] ) ? v - 1 : v;
+        c[d] = v;
+    }
+    return ( (int64_t)c[0] * g.nx[1] + c[1] ) * g.nx[2] + c[2];
+}
+
+__global__ __launch_bounds__( BLOCK ) void lcl_count_kernel(
+    PFieldDev pos, int64_t begin, int64_t end, CartGridDev g,
+    int32_t* counts, int32_t* cell_of, unsigned long long* bad )
+{
+    const int64_t p = begin + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( p >= end )
+        return;
+    const int64_t b = pos.base( p );
+    const double x[3] = { *pos.ptr<const double>( b, 0 ),
+                          *pos.ptr<const double>( b, 1 ),
+                          *pos.ptr<const double>( b, 2 ) };
+    const int64_t c = locate_cell( g, x );
+    if ( c < 0 )
+    {
+        atomicAdd( bad, 1ull );
+        cell_of[p - begin] = -1;
+        return;
+    }
+    cell_of[p - begin] = (int32_t)c;
+    atomicAdd( counts + c, 1 );
+}
+
+constexpr int SCAN_BLOCK = 1024;
+
+__global__ __launch_bounds__( SCAN_BLOCK ) void scan_reduce_kernel(
+    const int32_t* in, int64_t n, int64_t* block_sums )
+{
+    __shared__ int64_t red[SCAN_BLOCK / 64];
+    const int64_t i = (int64_t)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    int64_t v = ( i < n ) ? in[i] : 0;
+    for ( int o = 32; o > 0; o >>= 1 )
+        v += __shfl_down( v, o, 64 );
+    if ( ( threadIdx.x & 63 ) == 0 )
+        red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+    {
+        int64_t s = 0;
+        for ( int w = 0; w < SCAN_BLOCK / 64; ++w )
+            s += red[w];
+        block_sums[blockIdx.x] = s;
+    }
+}
+
+// Exclusive scan of the block sums by one workgroup (sequential per thread
+// chunk, then a workgroup scan of the chunk totals).
+__global__ __launch_bounds__( SCAN_BLOCK ) void scan_block_sums_kernel(
+    int64_t* block_sums, int64_t nb )
+{
+    __shared__ int64_t part[SCAN_BLOCK];
+    const int64_t chunk = ( nb + SCAN_BLOCK - 1 ) / SCAN_BLOCK;
+    const int64_t lo = threadIdx.x * chunk;
+    const int64_t hi = lo + chunk < nb ? lo + chunk : nb;
+    int64_t s = 0;
+    for ( int64_t i = lo; i < hi; ++i )
+        s += block_sums[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+    {
+        int64_t run = 0;
+        for ( int t = 0; t < SCAN_BLOCK; ++t )
+        {
+            const int64_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    int64_t run = part[threadIdx.x];
+    for ( int64_t i = lo; i < hi; ++i )
+    {
+        const int64_t v = block_sums[i];
+        block_sums[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__( SCAN_BLOCK ) void scan_apply_kernel(
+    const int32_t* in, int64_t* out, int64_t n, const int64_t* block_sums )
+{
+    __shared__ int64_t wsum[SCAN_BLOCK / 64];
+    const int64_t i = (int64_t)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t x = ( i < n ) ? in[i] : 0;
+    // Inclusive wave scan.
+    int64_t v = x;
+    for ( int o = 1; o < 64; o <<= 1 )
+    {
+        const int64_t u = __shfl_up( v, o, 64 );
+        if ( lane >= o )
+            v += u;
+    }
+    if ( lane == 63 )
+        wsum[wv] = v;
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+    {
+        int64_t run = 0;
+        for ( int w = 0; w < SCAN_BLOCK / 64; ++w )
+        {
+            const int64_t t = wsum[w];
+            wsum[w] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+    if ( i < n )
+        out[i] = block_sums[blockIdx.x] + wsum[wv] + v - x;
+}
+
+__global__ __launch_bounds__( BLOCK ) void lcl_place_kernel(
+    int64_t begin, int64_t end, const int32_t* cell_of, int64_t* cursor,
+    int64_t* permutes )
+{
+    const int64_t p = begin + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( p >= end )
+        return;
+    const int32_t c = cell_of[p - begin];
+    if ( c < 0 )
+        return;
+    const unsigned long long slot = atomicAdd(
+        reinterpret_cast<unsigned long long*>( cursor + c ), 1ull );
+    permutes[slot] = p;
+}
+
+// Restores ascending particle order inside every bin (the order a Serial
+// build produces), after the atomic placement above.
+__global__ __launch_bounds__( BLOCK ) void lcl_order_bins_kernel(
+    const int64_t* offsets, const int32_t* counts, int64_t ncell,
+    int64_t* permutes )
+{
+    const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( c >= ncell )
+        return;
+    const int n = counts[c];
+    if ( n < 2 )
+        return;
+    int64_t* q = permutes + offsets[c];
+    for ( int a = 1; a < n; ++a )
+    {
+        const int64_t v = q[a];
+        int b = a - 1;
+        while ( b >= 0 && q[b] > v )
+        {
+            q[b + 1] = q[b];
+            --b;
+        }
+        q[b + 1] = v;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__( BLOCK ) void permute_gather_kernel(
+    PFieldDev f, const int64_t* permutes, int64_t begin, int64_t end,
+    T* scratch )
+{
+    const int64_t i = begin + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( i >= end )
+        return;
+    const int64_t src = permutes[i - begin];
+    const int64_t b = f.base( src );
+    for ( int c = 0; c < f.ncomp; ++c )
+        scratch[( i - begin ) * f.ncomp + c] = *f.ptr<const T>( b, c );
+}
+
+template <class T>
+__global__ __launch_bounds__( BLOCK ) void permute_copy_back_kernel(
+    PFieldDev f, int64_t begin, int64_t end, const T* scratch )
+{
+    const int64_t i = begin + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( i >= end )
+        return;
+    const int64_t b = f.base( i );
+    for ( int c = 0; c < f.ncomp; ++c )
+        *f.ptr<T>( b, c ) = scratch[( i - begin ) * f.ncomp + c];
+}
+
+__global__ __launch_bounds__( BLOCK ) void fill_i32_kernel( int32_t* dst,
+                                                            int64_t n,
+                                                            int32_t value )
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( i < n )
+        dst[i] = value;
+}
+
+__global__ __launch_bounds__( BLOCK ) void copy_i64_kernel( const int64_t* src,
+                                                            int64_t* dst,
+                                                            int64_t n )
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( i < n )
+        dst[i] = src[i];
+}
+
+} // namespace
+
+hipError_t launch_halo_gather( Launcher& L, bool fp64, const MeshDev& m,
+                               const GFieldDev& g, int width, hipStream_t s )
+{
+    const int64_t n = (int64_t)g.ext[0] * g.ext[1] * g.ext[2];
+    L.before( "halo_gather", s );
+    if ( fp64 )
+        hipLaunchKernelGGL( halo_fill_kernel<double>, blocks_for( n ),
+                            dim3( BLOCK ), 0, s, m, g, width, 0 );
+    else
+        hipLaunchKernelGGL( halo_fill_kernel<float>, blocks_for( n ),
+                            dim3( BLOCK ), 0, s, m, g, width, 0 );
+    L.after( "halo_gather", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_halo_adjoint_fill( Launcher& L, bool fp64, const MeshDev& m,
+                                     const GFieldDev& g, int width,
+                                     hipStream_t s )
+{
+    const int64_t n = (int64_t)g.ext[0] * g.ext[1] * g.ext[2];
+    L.before( "halo_adjoint_fill", s );
+    if ( fp64 )
+        hipLaunchKernelGGL( halo_fill_kernel<double>, blocks_for( n ),
+                            dim3( BLOCK ), 0, s, m, g, width, 1 );
+    else
+        hipLaunchKernelGGL( halo_fill_kernel<float>, blocks_for( n ),
+                            dim3( BLOCK ), 0, s, m, g, width, 1 );
+    L.after( "halo_adjoint_fill", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_halo_scatter_sum( Launcher& L, bool fp64, const MeshDev& m,
+                                    const GFieldDev& g, int width,
+                                    hipStream_t s )
+{
+    int64_t n = 1;
+    for ( int d = 0; d < 3; ++d )
+        n *= m.num_cell[d] + ( m.periodic[d] ? 0 : 1 );
+    L.before( "halo_scatter_sum", s );
+    if ( fp64 )
+        hipLaunchKernelGGL( halo_fold_kernel<double>, blocks_for( n ),
+                            dim3( BLOCK ), 0, s, m, g, width );
+    else
+        hipLaunchKernelGGL( halo_fold_kernel<float>, blocks_for( n ),
+                            dim3( BLOCK ), 0, s, m, g, width );
+    L.after( "halo_scatter_sum", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_ghosts( Launcher& L, bool fp64, const MeshDev& m,
+                               const GFieldDev& g, hipStream_t s )
+{
+    const int64_t n = (int64_t)g.ext[0] * g.ext[1] * g.ext[2];
+    L.before( "zero_ghosts", s );
+    if ( fp64 )
+        hipLaunchKernelGGL( zero_ghosts_kernel<double>, blocks_for( n ),
+                            dim3( BLOCK ), 0, s, m, g );
+    else
+        hipLaunchKernelGGL( zero_ghosts_kernel<float>, blocks_for( n ),
+                            dim3( BLOCK ), 0, s, m, g );
+    L.after( "zero_ghosts", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_lcl_count( Launcher& L, const PFieldDev& pos, int64_t begin,
+                             int64_t end, const CartGridDev& g,
+                             int32_t* counts, int32_t* cell_of,
+                             unsigned long long* bad, hipStream_t s )
+{
+    if ( end <= begin )
+        return hipSuccess;
+    L.before( "lcl_count", s );
+    hipLaunchKernelGGL( lcl_count_kernel, blocks_for( end - begin ),
+                        dim3( BLOCK ), 0, s, pos, begin, end, g, counts,
+                        cell_of, bad );
+    L.after( "lcl_count", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_exclusive_scan( Launcher& L, const int32_t* in, int64_t* out,
+                                  int64_t n, int64_t* block_sums,
+                                  int64_t nblock_cap, hipStream_t s )
+{
+    if ( n <= 0 )
+        return hipSuccess;
+    const int64_t nb = ( n + SCAN_BLOCK - 1 ) / SCAN_BLOCK;
+    if ( nb > nblock_cap )
+        return hipErrorInvalidValue;
+    L.before( "exclusive_scan", s );
+    hipLaunchKernelGGL( scan_reduce_kernel, dim3( (unsigned)nb ),
+                        dim3( SCAN_BLOCK ), 0, s, in, n, block_sums );
+    hipLaunchKernelGGL( scan_block_sums_kernel, dim3( 1 ), dim3( SCAN_BLOCK ),
+                        0, s, block_sums, nb );
+    hipLaunchKernelGGL( scan_apply_kernel, dim3( (unsigned)nb ),
+                        dim3( SCAN_BLOCK ), 0, s, in, out, n, block_sums );
+    L.after( "exclusive_scan", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_lcl_place( Launcher& L, int64_t begin, int64_t end,
+                             const int32_t* cell_of, int64_t* cursor,
+                             int64_t* permutes, hipStream_t s )
+{
+    if ( end <= begin )
+        return hipSuccess;
+    L.before( "lcl_place", s );
+    hipLaunchKernelGGL( lcl_place_kernel, blocks_for( end - begin ),
+                        dim3( BLOCK ), 0, s, begin, end, cell_of, cursor,
+                        permutes );
+    L.after( "lcl_place", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_lcl_order_bins( Launcher& L, const int64_t* offsets,
+                                  const int32_t* counts, int64_t ncell,
+                                  int64_t* permutes, hipStream_t s )
+{
+    if ( ncell <= 0 )
+        return hipSuccess;
+    L.before( "lcl_order_bins", s );
+    hipLaunchKernelGGL( lcl_order_bins_kernel, blocks_for( ncell ),
+                        dim3( BLOCK ), 0, s, offsets, counts, ncell,
+                        permutes );
+    L.after( "lcl_order_bins", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_permute_gather( Launcher& L, const PFieldDev& f,
+                                  int elem_size, const int64_t* permutes,
+                                  int64_t begin, int64_t end, void* scratch,
+                                  hipStream_t s )
+{
+    if ( end <= begin )
+        return hipSuccess;
+    L.before( "permute_gather", s );
+    if ( elem_size == 8 )
+        hipLaunchKernelGGL( permute_gather_kernel<uint64_t>,
+                            blocks_for( end - begin ), dim3( BLOCK ), 0, s, f,
+                            permutes, begin, end,
+                            reinterpret_cast<uint64_t*>( scratch ) );
+    else
+        hipLaunchKernelGGL( permute_gather_kernel<uint32_t>,
+                            blocks_for( end - begin ), dim3( BLOCK ), 0, s, f,
+                            permutes, begin, end,
+                            reinterpret_cast<uint32_t*>( scratch ) );
+    L.after( "permute_gather", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_permute_copy_back( Launcher& L, const PFieldDev& f,
+                                     int elem_size, int64_t begin,
+                                     int64_t end, const void* scratch,
+                                     hipStream_t s )
+{
+    if ( end <= begin )
+        return hipSuccess;
+    L.before( "permute_copy_back", s );
+    if ( elem_size == 8 )
+        hipLaunchKernelGGL( permute_copy_back_kernel<uint64_t>,
+                            blocks_for( end - begin ), dim3( BLOCK ), 0, s, f,
+                            begin, end,
+                            reinterpret_cast<const uint64_t*>( scratch ) );
+    else
+        hipLaunchKernelGGL( permute_copy_back_kernel<uint32_t>,
+                            blocks_for( end - begin ), dim3( BLOCK ), 0, s, f,
+                            begin, end,
+                            reinterpret_cast<const uint32_t*>( scratch ) );
+    L.after( "permute_copy_back", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32( Launcher& L, int32_t* dst, int64_t n,
+                            int32_t value, hipStream_t s )
+{
+    if ( n <= 0 )
+        return hipSuccess;
+    L.before( "fill_i32", s );
+    hipLaunchKernelGGL( fill_i32_kernel, blocks_for( n ), dim3( BLOCK ), 0, s,
+                        dst, n, value );
+    L.after( "fill_i32", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_i64( Launcher& L, const int64_t* src, int64_t* dst,
+                            int64_t n, hipStream_t s )
+{
+    if ( n <= 0 )
+        return hipSuccess;
+    L.before( "copy_i64", s );
+    hipLaunchKernelGGL( copy_i64_kernel, blocks_for( n ), dim3( BLOCK ), 0, s,
+                        src, dst, n );
+    L.after( "copy_i64", s );
+    return hipGetLastError();
+}
+
+} // namespace cabana_amd

@@ -1,0 +1,249 @@
+// Internal types shared by the C-ABI layer and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cabana_amd.h"
+
+namespace cabana_amd
+{
+
+// Mesh constants as the kernels see them.  low/rdx follow evaluateSpline
+// (Cabana_Grid_Splines.hpp:1283-1342): low = coordinate of ghosted node 0,
+// dx = coordinate(node 1) - coordinate(node 0), rdx = 1 / dx.
+struct MeshDev
+{
+    double low[3];
+    double rdx[3];
+    int32_t num_node[3]; // ghosted node extent
+    int32_t num_cell[3];
+    int32_t halo;        // halo cell width of the local grid
+    int32_t periodic[3];
+};
+
+// Particle field addressing (Cabana_Slice.hpp:119-190).  For a power-of-two
+// vector length the block and lane come from shift/mask; when one block holds
+// every particle, shift = 62 so the block index is always 0.
+struct PFieldDev
+{
+    char* data;
+    int64_t vl;
+    int64_t stride;
+    int32_t shift; // -1: general vector length (division)
+    int32_t ncomp;
+    int64_t mask;
+
+    __device__ __forceinline__ int64_t base( int64_t p ) const
+    {
+        if ( shift >= 0 )
+            return ( p >> shift ) * stride + ( p & mask );
+        return ( p / vl ) * stride + ( p % vl );
+    }
+    template <class T>
+    __device__ __forceinline__ T* ptr( int64_t b, int c ) const
+    {
+        return reinterpret_cast<T*>( data ) + b + vl * c;
+    }
+};
+
+struct GFieldDev
+{
+    char* data;
+    int64_t s[4];
+    int32_t ext[4];
+};
+
+// Bilinear forms B(P, G; W, dW) between a particle vector P and a node vector
+// G.  P2G scatters dB/dG, G2P gathers dB/dP and the position adjoint
+// differentiates B through the weights.
+enum Form : int32_t
+{
+    FORM_SV = 0,  // P[1]  G[1]: P0 G0 W
+    FORM_VV = 1,  // P[3]  G[3]: sum_d Pd Gd W
+    FORM_SG = 2,  // P[1]  G[3]: sum_d P0 Gd dW_d
+    FORM_VD = 3,  // P[3]  G[1]: sum_d Pd G0 dW_d
+    FORM_TD = 4,  // P[9]  G[3]: sum P[3a+b] G[a] dW_b
+    FORM_TDT = 5, // P[9]  G[3]: sum P[3a+b] G[b] dW_a
+    FORM_COUNT = 6
+};
+
+// Forward operator -> form.  P2G: ScalarValue->SV, VectorValue->VV,
+// ScalarGradient->SG, VectorDivergence->VD, TensorDivergence->TD.
+// G2P: ScalarValue->SV, VectorValue->VV, ScalarGradient->VD,
+// VectorGradient->TDT, VectorDivergence->SG.
+inline int p2g_form( int op )
+{
+    static const int f[5] = { FORM_SV, FORM_VV, FORM_SG, FORM_VD, FORM_TD };
+    return f[op];
+}
+inline int g2p_form( int op )
+{
+    static const int f[5] = { FORM_SV, FORM_VV, FORM_VD, FORM_TDT, FORM_SG };
+    return f[op];
+}
+inline int form_pcomp( int form )
+{
+    static const int n[6] = { 1, 3, 1, 3, 9, 9 };
+    return n[form];
+}
+inline int form_gdof( int form )
+{
+    static const int n[6] = { 1, 3, 3, 1, 3, 3 };
+    return n[form];
+}
+
+constexpr int MAX_TERMS = 3;
+
+// One term of a scatter pass (P2G forward, or G2P adjoint):
+//   grid += mult * dB/dG (P = pin)
+//   xbar += mult * dB/dx (P = pin, G = gx)     [only when the pass has xbar]
+struct ScatterTermDev
+{
+    PFieldDev pin;
+    GFieldDev grid;
+    GFieldDev gx;
+    double mult;
+    int32_t form;
+};
+
+// One term of a gather pass (G2P forward, or P2G adjoint):
+//   pout += mult * dB/dP (G = grid)
+//   xbar += mult * dB/dx (P = px, G = grid)     [only when the pass has xbar]
+struct GatherTermDev
+{
+    PFieldDev pout;
+    PFieldDev px;
+    GFieldDev grid;
+    double mult;
+    int32_t form;
+};
+
+struct ScatterArgs
+{
+    MeshDev mesh;
+    PFieldDev points;
+    PFieldDev xbar;
+    int64_t begin;
+    int64_t end;
+    int32_t nterms;
+    int32_t has_xbar;
+    ScatterTermDev t[MAX_TERMS];
+    unsigned long long* skipped;
+};
+
+struct GatherArgs
+{
+    MeshDev mesh;
+    PFieldDev points;
+    PFieldDev xbar;
+    int64_t begin;
+    int64_t end;
+    int32_t nterms;
+    int32_t has_xbar;
+    GatherTermDev t[MAX_TERMS];
+    unsigned long long* skipped;
+};
+
+// Tile description for the cell-sorted scatter: a brick of cells whose
+// particles are read through cell offsets.
+struct TileArgs
+{
+    const int64_t* cell_offsets; // exclusive scan, per mesh cell
+    const int32_t* cell_counts;
+    int32_t tile[3];             // cells per tile and dimension
+    int32_t ntile[3];
+    int32_t win[3];              // window nodes per dimension
+    int32_t tiles_per_xcd;       // blocks are dealt round-robin to 8 XCDs
+    int32_t flush_perm[3];       // grid dims from largest to smallest stride
+    int32_t max_runs;            // tile[0] * tile[1]
+    int64_t bin_begin;           // particle index of cell offset 0
+};
+
+// Launch-time error/timing context owned by the C-ABI layer.
+class Launcher
+{
+  public:
+    virtual ~Launcher() = default;
+    // Called around every kernel launch; name is the kernel family.
+    virtual void before( const char* name, hipStream_t s ) = 0;
+    virtual void after( const char* name, hipStream_t s ) = 0;
+};
+
+// Fused term sets with their own kernels; any other multi-term call runs one
+// single-form pass per term.
+enum Plan : int32_t
+{
+    PLAN_SINGLE = 0,  // one term, any form
+    PLAN_MPM_P2G = 1, // (SV, VV, TD): mass, momentum, stress divergence
+    PLAN_MPM_G2P = 2  // (VV, TDT) on one shared grid: velocity + gradient
+};
+
+// Kernel launchers (return hipSuccess or the launch error), one set per
+// spline order (compiled in separate translation units).
+#define CABANA_AMD_DECLARE_LAUNCHERS( SUFFIX )                                 \
+    hipError_t launch_gather_##SUFFIX( Launcher& L, bool fp64, int plan,       \
+                                       const GatherArgs& a, hipStream_t s );   \
+    hipError_t launch_scatter_atomic_##SUFFIX( Launcher& L, bool fp64,         \
+                                               int plan, const ScatterArgs& a, \
+                                               hipStream_t s );                \
+    hipError_t launch_scatter_tiled_##SUFFIX( Launcher& L, bool fp64,          \
+                                              int plan, const ScatterArgs& a,  \
+                                              const TileArgs& t,               \
+                                              hipStream_t s );
+CABANA_AMD_DECLARE_LAUNCHERS( o1 )
+CABANA_AMD_DECLARE_LAUNCHERS( o2 )
+CABANA_AMD_DECLARE_LAUNCHERS( o3 )
+
+// Halo kernels on a ghosted node grid.
+hipError_t launch_halo_gather( Launcher& L, bool fp64, const MeshDev& m,
+                               const GFieldDev& g, int width, hipStream_t s );
+hipError_t launch_halo_scatter_sum( Launcher& L, bool fp64, const MeshDev& m,
+                                    const GFieldDev& g, int width,
+                                    hipStream_t s );
+// Transpose of the fold for a cotangent grid: ghost := owner within the band,
+// 0 for every other non-owned node.
+hipError_t launch_halo_adjoint_fill( Launcher& L, bool fp64, const MeshDev& m,
+                                     const GFieldDev& g, int width,
+                                     hipStream_t s );
+// Zero every non-owned node.
+hipError_t launch_zero_ghosts( Launcher& L, bool fp64, const MeshDev& m,
+                               const GFieldDev& g, hipStream_t s );
+
+// Binning.
+struct CartGridDev
+{
+    double min[3];
+    double rdx[3];
+    int32_t nx[3];
+};
+hipError_t launch_lcl_count( Launcher& L, const PFieldDev& pos, int64_t begin,
+                             int64_t end, const CartGridDev& g,
+                             int32_t* counts, int32_t* cell_of,
+                             unsigned long long* bad, hipStream_t s );
+hipError_t launch_exclusive_scan( Launcher& L, const int32_t* in, int64_t* out,
+                                  int64_t n, int64_t* block_sums,
+                                  int64_t nblock_cap, hipStream_t s );
+hipError_t launch_lcl_place( Launcher& L, int64_t begin, int64_t end,
+                             const int32_t* cell_of, int64_t* cursor,
+                             int64_t* permutes, hipStream_t s );
+hipError_t launch_lcl_order_bins( Launcher& L, const int64_t* offsets,
+                                  const int32_t* counts, int64_t ncell,
+                                  int64_t* permutes, hipStream_t s );
+hipError_t launch_permute_gather( Launcher& L, const PFieldDev& f,
+                                  int elem_size, const int64_t* permutes,
+                                  int64_t begin, int64_t end, void* scratch,
+                                  hipStream_t s );
+hipError_t launch_permute_copy_back( Launcher& L, const PFieldDev& f,
+                                     int elem_size, int64_t begin,
+                                     int64_t end, const void* scratch,
+                                     hipStream_t s );
+hipError_t launch_copy_i64( Launcher& L, const int64_t* src, int64_t* dst,
+                            int64_t n, hipStream_t s );
+hipError_t launch_fill_i32( Launcher& L, int32_t* dst, int64_t n,
+                            int32_t value, hipStream_t s );
+
+} // namespace cabana_amd

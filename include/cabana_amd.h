@@ -1,0 +1,332 @@
+/*
+ * cabana_amd: MI355X-native particle<->grid interpolation (P2G / G2P) with
+ * Cabana's operator semantics, behind a plain C ABI.
+ *
+ * All pointers are device pointers unless stated otherwise.  Every entry
+ * point is stream ordered and asynchronous; nothing here allocates or
+ * copies user data.  Calls return a status code and never throw.
+ *
+ * Reference interfaces (paths relative to the Cabana source tree):
+ *   grid/src/Cabana_Grid_Interpolation.hpp   p2g / g2p drivers and functors
+ *   grid/src/Cabana_Grid_Splines.hpp         Spline<1..3>, evaluateSpline
+ *   grid/src/Cabana_Grid_HaloBase.hpp,
+ *   grid/src/impl/Cabana_Grid_Halo_Mpi.hpp   node halo gather / scatter(Sum)
+ *   core/src/Cabana_LinkedCellList.hpp,
+ *   core/src/Cabana_Sort.hpp                 cell binning and permute
+ *   core/src/Cabana_Slice.hpp                AoSoA slice addressing
+ */
+#ifndef CABANA_AMD_H
+#define CABANA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CABANA_AMD_ABI_VERSION 1
+
+typedef struct cabana_amd_context cabana_amd_context;
+
+/* A hipStream_t (0 = the legacy default stream). */
+typedef void* cabana_amd_stream;
+
+typedef enum
+{
+    CABANA_AMD_OK = 0,
+    CABANA_AMD_ERROR_INVALID_ARGUMENT = 1,
+    CABANA_AMD_ERROR_UNSUPPORTED = 2,
+    CABANA_AMD_ERROR_OUT_OF_MEMORY = 3,
+    CABANA_AMD_ERROR_DEVICE = 4,
+    CABANA_AMD_ERROR_OUT_OF_DOMAIN = 5
+} cabana_amd_status;
+
+typedef enum
+{
+    CABANA_AMD_FLOAT64 = 0,
+    CABANA_AMD_FLOAT32 = 1,
+    CABANA_AMD_INT32 = 2,
+    CABANA_AMD_INT64 = 3
+} cabana_amd_dtype;
+
+/* Point-to-grid operators: the functors built by createScalarValueP2G,
+ * createVectorValueP2G, createScalarGradientP2G, createVectorDivergenceP2G
+ * and createTensorDivergenceP2G (Cabana_Grid_Interpolation.hpp:1324-1566). */
+typedef enum
+{
+    CABANA_AMD_SCALAR_VALUE_P2G = 0,
+    CABANA_AMD_VECTOR_VALUE_P2G = 1,
+    CABANA_AMD_SCALAR_GRADIENT_P2G = 2,
+    CABANA_AMD_VECTOR_DIVERGENCE_P2G = 3,
+    CABANA_AMD_TENSOR_DIVERGENCE_P2G = 4
+} cabana_amd_p2g_op;
+
+/* Grid-to-point operators: createScalarValueG2P, createVectorValueG2P,
+ * createScalarGradientG2P, createVectorGradientG2P, createVectorDivergenceG2P
+ * (Cabana_Grid_Interpolation.hpp:938-1182). */
+typedef enum
+{
+    CABANA_AMD_SCALAR_VALUE_G2P = 0,
+    CABANA_AMD_VECTOR_VALUE_G2P = 1,
+    CABANA_AMD_SCALAR_GRADIENT_G2P = 2,
+    CABANA_AMD_VECTOR_GRADIENT_G2P = 3,
+    CABANA_AMD_VECTOR_DIVERGENCE_G2P = 4
+} cabana_amd_g2p_op;
+
+/* One rank's uniform mesh, Node entity: createUniformGlobalMesh(low, high,
+ * cell_size) + createGlobalGrid(periodic) + createLocalGrid(halo_cell_width)
+ * (Cabana_Grid_GlobalMesh.hpp:59-102, Cabana_Grid_LocalGrid.hpp:607).  The
+ * number of cells per dimension is rint((high - low) / cell_size). */
+typedef struct
+{
+    double low_corner[3];
+    double high_corner[3];
+    double cell_size[3];
+    int32_t halo_cell_width;
+    int32_t periodic[3];
+} cabana_amd_mesh;
+
+/* Particle field (a Cabana slice or a Kokkos View indexed (p, d0[, d1])).
+ * Element (p, c), with c = d0 * D1 + d1, is at
+ *     data[(p / vector_length) * soa_stride + (p % vector_length)
+ *          + vector_length * c]
+ * which is Cabana_Slice.hpp:119-190.  An AoSoA slice has vector_length = VL
+ * and soa_stride = slice.stride(0); an (n, ncomp) row-major array has
+ * vector_length = 1, soa_stride = ncomp; a column-major one has
+ * vector_length = soa_stride = n. */
+typedef struct
+{
+    void* data;
+    int64_t num_particle;
+    int32_t num_comp;
+    int32_t dtype; /* cabana_amd_dtype */
+    int64_t vector_length;
+    int64_t soa_stride;
+} cabana_amd_particle_field;
+
+/* Node array over the ghosted local index space (Array::view(),
+ * Cabana_Grid_Array.hpp:227-304): extent = (n_i, n_j, n_k, dofs) with
+ * n_d = num_cell_d + 2 * halo_cell_width + 1; strides are in elements, so
+ * both LayoutLeft and LayoutRight views are accepted. */
+typedef struct
+{
+    void* data;
+    int32_t extent[4];
+    int64_t stride[4];
+    int32_t dtype; /* cabana_amd_dtype */
+} cabana_amd_grid_field;
+
+/* Cell binning of the particles (the output of
+ * cabana_amd_linked_cell_build followed by cabana_amd_permute).  When passed
+ * to the interpolation calls, particles [begin, end) must be stored in
+ * cardinal cell order (i slowest, k fastest) over the mesh's own cells, which
+ * lets the kernels walk cell-sorted tiles.  Particles outside [begin, end)
+ * are still processed.  NULL means "no ordering is known". */
+typedef struct
+{
+    const int32_t* counts;  /* per cell */
+    const int64_t* offsets; /* exclusive scan of counts */
+    int64_t begin;
+    int64_t end;
+    int32_t num_bin[3];     /* must equal the mesh's cells per dimension */
+} cabana_amd_binning;
+
+typedef struct
+{
+    int32_t op; /* cabana_amd_p2g_op */
+    double multiplier;
+    cabana_amd_particle_field values; /* read */
+    cabana_amd_grid_field grid;       /* accumulated into, then folded */
+} cabana_amd_p2g_term;
+
+typedef struct
+{
+    int32_t op; /* cabana_amd_g2p_op */
+    double multiplier;
+    cabana_amd_grid_field grid;       /* ghosts filled, then read */
+    cabana_amd_particle_field values; /* accumulated into */
+} cabana_amd_g2p_term;
+
+/* ----------------------------------------------------------------------- */
+/* Context                                                                  */
+/* ----------------------------------------------------------------------- */
+
+/* Creates a context bound to a HIP device.  The context owns reusable device
+ * scratch (sort keys, tile lists) and per-call diagnostics. */
+cabana_amd_status cabana_amd_context_create( int32_t device,
+                                             cabana_amd_context** ctx );
+cabana_amd_status cabana_amd_context_destroy( cabana_amd_context* ctx );
+
+/* Text of the last error seen by this context (or by the calling thread
+ * when ctx is NULL).  Never NULL. */
+const char* cabana_amd_last_error( const cabana_amd_context* ctx );
+
+int32_t cabana_amd_abi_version( void );
+
+/* Waits for the stream, then reports and clears the count of particles that
+ * were skipped because their spline stencil left the ghosted grid (Cabana
+ * has no such check: such particles are undefined behaviour there).
+ * Returns CABANA_AMD_ERROR_OUT_OF_DOMAIN when the count is non-zero. */
+cabana_amd_status cabana_amd_check_domain( cabana_amd_context* ctx,
+                                           cabana_amd_stream stream,
+                                           int64_t* num_skipped );
+
+/* ----------------------------------------------------------------------- */
+/* Interpolation                                                            */
+/* ----------------------------------------------------------------------- */
+
+/* p2g(functor, points, num_point, Spline<order>, halo, array)
+ * (Cabana_Grid_Interpolation.hpp:1217-1313) for one or more functors in one
+ * pass over the particles.  Each term ADDS its contributions into its grid
+ * (owned and ghost nodes), then every distinct grid is folded once: the
+ * ghost nodes within halo_width are summed into their periodic owners
+ * (Halo::scatter with ScatterReduce::Sum).  Ghost nodes keep their partial
+ * sums.  halo_width < 0 means the full halo cell width.  The spline order is
+ * 1, 2 or 3; points are FLOAT64 (num_comp 3).  Precision: FLOAT64 values and
+ * grids give the plain FP64 path; FLOAT32 values and grids give the mixed
+ * path (FP64 cell-local offsets, FP32 weights and accumulation). */
+cabana_amd_status cabana_amd_p2g( cabana_amd_context* ctx,
+                                  const cabana_amd_mesh* mesh,
+                                  int32_t spline_order,
+                                  const cabana_amd_particle_field* points,
+                                  int32_t num_terms,
+                                  const cabana_amd_p2g_term* terms,
+                                  int32_t halo_width,
+                                  const cabana_amd_binning* bins,
+                                  cabana_amd_stream stream );
+
+/* g2p(array, halo, points, num_point, Spline<order>, functor)
+ * (Cabana_Grid_Interpolation.hpp:831-927) for one or more functors.  Every
+ * distinct grid first gets its ghosts filled from their periodic owners
+ * (Halo::gather), then each term ADDS multiplier * result into its particle
+ * field. */
+cabana_amd_status cabana_amd_g2p( cabana_amd_context* ctx,
+                                  const cabana_amd_mesh* mesh,
+                                  int32_t spline_order,
+                                  const cabana_amd_particle_field* points,
+                                  int32_t num_terms,
+                                  const cabana_amd_g2p_term* terms,
+                                  int32_t halo_width,
+                                  const cabana_amd_binning* bins,
+                                  cabana_amd_stream stream );
+
+/* ----------------------------------------------------------------------- */
+/* Adjoints (no reference counterpart: Cabana has no differentiation)       */
+/* ----------------------------------------------------------------------- */
+
+typedef struct
+{
+    int32_t op; /* cabana_amd_p2g_op */
+    double multiplier;
+    cabana_amd_particle_field values;     /* forward input (read) */
+    cabana_amd_grid_field grid_bar;       /* cotangent of the folded grid */
+    cabana_amd_particle_field values_bar; /* += dL/dvalues; data may be NULL */
+} cabana_amd_p2g_adjoint_term;
+
+typedef struct
+{
+    int32_t op; /* cabana_amd_g2p_op */
+    double multiplier;
+    cabana_amd_grid_field grid;           /* forward input (ghosts filled) */
+    cabana_amd_particle_field values_bar; /* cotangent of the output */
+    cabana_amd_grid_field grid_bar;       /* += dL/dgrid; data may be NULL */
+} cabana_amd_g2p_adjoint_term;
+
+/* Adjoint of cabana_amd_p2g: a G2P-shaped gather.  Only the owned nodes of
+ * each grid_bar are read as the cotangent; its ghost nodes are overwritten
+ * with the transpose of the fold (owned value copied into each periodic
+ * ghost within halo_width, zero elsewhere).  values_bar += dL/dvalues and,
+ * when points_bar is not NULL, points_bar (FLOAT64, 3 comps) += dL/dpoints
+ * through the spline weight derivatives.  Weights are recomputed. */
+cabana_amd_status cabana_amd_p2g_adjoint(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+    int32_t spline_order, const cabana_amd_particle_field* points,
+    int32_t num_terms, const cabana_amd_p2g_adjoint_term* terms,
+    int32_t halo_width, const cabana_amd_particle_field* points_bar,
+    const cabana_amd_binning* bins, cabana_amd_stream stream );
+
+/* Adjoint of cabana_amd_g2p: a P2G-shaped scatter.  The ghosts of each
+ * grid are filled as in the forward call.  Each distinct grid_bar has its
+ * ghost nodes zeroed, receives the scattered cotangents, and is folded, so
+ * its owned nodes get += dL/dgrid.  points_bar (optional) += dL/dpoints. */
+cabana_amd_status cabana_amd_g2p_adjoint(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+    int32_t spline_order, const cabana_amd_particle_field* points,
+    int32_t num_terms, const cabana_amd_g2p_adjoint_term* terms,
+    int32_t halo_width, const cabana_amd_particle_field* points_bar,
+    const cabana_amd_binning* bins, cabana_amd_stream stream );
+
+/* ----------------------------------------------------------------------- */
+/* Halo (single rank, periodic dimensions exchange with themselves)          */
+/* ----------------------------------------------------------------------- */
+
+/* Halo::gather (Cabana_Grid_Halo_Mpi.hpp:76-157): ghost = periodic owner. */
+cabana_amd_status cabana_amd_halo_gather( cabana_amd_context* ctx,
+                                          const cabana_amd_mesh* mesh,
+                                          const cabana_amd_grid_field* grid,
+                                          int32_t halo_width,
+                                          cabana_amd_stream stream );
+
+/* Halo::scatter(ScatterReduce::Sum) (Cabana_Grid_Halo_Mpi.hpp:166-247):
+ * owner += every periodic ghost image of it. */
+cabana_amd_status cabana_amd_halo_scatter_sum(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+    const cabana_amd_grid_field* grid, int32_t halo_width,
+    cabana_amd_stream stream );
+
+/* ----------------------------------------------------------------------- */
+/* Binning                                                                  */
+/* ----------------------------------------------------------------------- */
+
+/* Host-side: number of bins per dimension of LinkedCellList's Cartesian grid
+ * (CartesianGrid::init, core/src/impl/Cabana_CartesianGrid.hpp:61-73). */
+cabana_amd_status cabana_amd_linked_cell_num_bins( const double grid_delta[3],
+                                                   const double grid_min[3],
+                                                   const double grid_max[3],
+                                                   int32_t num_bin[3] );
+
+/* LinkedCellList::build(positions, begin, end)
+ * (core/src/Cabana_LinkedCellList.hpp:651-739), reproducing the Serial
+ * result bit for bit: counts[c] = particles in cell c, offsets = exclusive
+ * scan, permutes[offsets[c] + r] = the r-th particle (ascending index) of
+ * cell c, as absolute particle indices.  counts and offsets hold one entry
+ * per bin; permutes holds end - begin entries.  Particles outside the
+ * Cartesian grid make the call return CABANA_AMD_ERROR_OUT_OF_DOMAIN (the
+ * reference indexes out of bounds). */
+cabana_amd_status cabana_amd_linked_cell_build(
+    cabana_amd_context* ctx, const cabana_amd_particle_field* positions,
+    int64_t begin, int64_t end, const double grid_delta[3],
+    const double grid_min[3], const double grid_max[3], int32_t* counts,
+    int64_t* offsets, int64_t* permutes, cabana_amd_stream stream );
+
+/* permute(binning_data, slice) (core/src/Cabana_Sort.hpp:600-656):
+ * field[i] = old field[permutes[i - begin]] for i in [begin, end), bitwise,
+ * for every component.  Any dtype. */
+cabana_amd_status cabana_amd_permute( cabana_amd_context* ctx,
+                                      const int64_t* permutes, int64_t begin,
+                                      int64_t end,
+                                      const cabana_amd_particle_field* field,
+                                      cabana_amd_stream stream );
+
+/* ----------------------------------------------------------------------- */
+/* Kernel timing (HIP events around every launch of this context)           */
+/* ----------------------------------------------------------------------- */
+
+cabana_amd_status cabana_amd_profile_enable( cabana_amd_context* ctx,
+                                             int32_t enable );
+cabana_amd_status cabana_amd_profile_reset( cabana_amd_context* ctx );
+/* Synchronizes the recorded events, then returns the number of distinct
+ * kernel names seen. */
+cabana_amd_status cabana_amd_profile_count( cabana_amd_context* ctx,
+                                            int32_t* count );
+cabana_amd_status cabana_amd_profile_entry( cabana_amd_context* ctx,
+                                            int32_t index, char* name,
+                                            int32_t name_capacity,
+                                            int64_t* launches,
+                                            double* total_ms );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
