@@ -55,9 +55,10 @@ def build_library(verbose=False):
     hipcc = _hipcc()
     jobs = []
     for order in (1, 2, 3):
-        obj = os.path.join(BUILD, "ca_kernels_o%d.o" % order)
-        jobs.append([hipcc] + COMMON + ["-DCA_ORDER=%d" % order, "-c",
-                                        os.path.join(CSRC, "ca_kernels.hip"), "-o", obj])
+        for src in ("ca_kernels", "ca_tiled"):
+            obj = os.path.join(BUILD, "%s_o%d.o" % (src, order))
+            jobs.append([hipcc] + COMMON + ["-DCA_ORDER=%d" % order, "-c",
+                                            os.path.join(CSRC, src + ".hip"), "-o", obj])
     jobs.append([hipcc] + COMMON + ["-c", os.path.join(CSRC, "ca_grid.hip"), "-o",
                                     os.path.join(BUILD, "ca_grid.o")])
     jobs.append([hipcc] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, "ca_api.cpp"),

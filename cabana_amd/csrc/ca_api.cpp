@@ -321,10 +321,19 @@ cabana_amd_status make_gfield( cabana_amd_context* ctx, const MeshDev& m,
         if ( g.stride[d] < 1 && g.extent[d] > 1 )
             return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                          std::string( what ) + ": non-positive stride" );
+    // Device addressing uses 32-bit element strides and byte offsets.
+    int64_t max_elem = 0;
+    for ( int d = 0; d < 4; ++d )
+        max_elem += ( g.extent[d] - 1 ) * g.stride[d];
+    const int64_t es = ( g.dtype == CABANA_AMD_FLOAT64 || g.dtype == CABANA_AMD_INT64 ) ? 8 : 4;
+    if ( max_elem >= ( int64_t( 1 ) << 31 ) ||
+         ( max_elem + 1 ) * es > ( int64_t( 1 ) << 32 ) )
+        return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED,
+                     std::string( what ) + ": grid arrays above 4 GiB are not supported" );
     out.data = static_cast<char*>( g.data );
     for ( int d = 0; d < 4; ++d )
     {
-        out.s[d] = g.stride[d];
+        out.s[d] = (int32_t)g.stride[d];
         out.ext[d] = g.extent[d];
     }
     return CABANA_AMD_OK;
@@ -397,26 +406,83 @@ hipError_t run_scatter_atomic( cabana_amd_context* ctx, int order, bool fp64,
     }
 }
 
-hipError_t run_scatter_tiled( cabana_amd_context* ctx, int order, bool fp64,
-                              int plan, const ScatterArgs& a,
-                              const TileArgs& t, hipStream_t s )
+hipError_t run_gather_tiled( cabana_amd_context* ctx, int order, bool fp64,
+                             int plan, const GatherArgs& a, const TileArgs& t,
+                             size_t lds, hipStream_t s )
 {
     switch ( order )
     {
     case 1:
-        return launch_scatter_tiled_o1( *ctx, fp64, plan, a, t, s );
+        return launch_gather_tiled_o1( *ctx, fp64, plan, a, t, lds, s );
     case 2:
-        return launch_scatter_tiled_o2( *ctx, fp64, plan, a, t, s );
+        return launch_gather_tiled_o2( *ctx, fp64, plan, a, t, lds, s );
     default:
-        return launch_scatter_tiled_o3( *ctx, fp64, plan, a, t, s );
+        return launch_gather_tiled_o3( *ctx, fp64, plan, a, t, lds, s );
     }
 }
 
-// Builds tile parameters for cell-sorted particles; returns false when the
-// binning does not describe the mesh's own cells.
-bool make_tiles( const MeshDev& m, int order, bool fp64, int nslot_dofs,
+hipError_t run_scatter_cells( cabana_amd_context* ctx, int order, bool fp64,
+                              int plan, const ScatterArgs& a,
+                              const TileArgs& t, size_t lds, hipStream_t s )
+{
+    switch ( order )
+    {
+    case 1:
+        return launch_scatter_cells_o1( *ctx, fp64, plan, a, t, lds, s );
+    case 2:
+        return launch_scatter_cells_o2( *ctx, fp64, plan, a, t, lds, s );
+    default:
+        return launch_scatter_cells_o3( *ctx, fp64, plan, a, t, lds, s );
+    }
+}
+
+size_t a16( size_t b ) { return ( b + 15 ) & ~(size_t)15; }
+
+enum TileKind
+{
+    TILE_GATHER = 0,
+    TILE_SCATTER = 1
+};
+
+// LDS bytes of the tiled kernels; mirrors their in-kernel layout.
+size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng,
+                       int nslot, int T, int order, int cap )
+{
+    const int extra = ( order == 1 ) ? 1 : 3;
+    const size_t w = T + extra;
+    const size_t wn = w * w * w;
+    const size_t runs = (size_t)T * T;
+    size_t b = 0;
+    if ( kind == TILE_GATHER )
+    {
+        const size_t real = fp64 ? 8 : 4;
+        for ( int i = 0; i < nslot; ++i )
+            b = a16( b + real * wn * ( slot_ng[i] == 1 ? 1 : 4 ) );
+        b = a16( b + 8 * runs );
+        b = a16( b + 4 * ( runs + 1 ) );
+        return b;
+    }
+    int dt = 0;
+    for ( int i = 0; i < nslot; ++i )
+        dt += slot_ng[i];
+    const size_t nbase = (size_t)( T + ( order == 2 ? 1 : 0 ) );
+    b = a16( 8 * (size_t)dt * wn );
+    b = a16( b + 4 * ( nbase * nbase * nbase + 1 ) );
+    b = a16( b + 4 * (size_t)cap );
+    b = a16( b + 4 * (size_t)cap );
+    b = a16( b + 8 * runs );
+    b = a16( b + 4 * ( runs + 1 ) );
+    b += 16;
+    return b;
+}
+
+// Tile parameters for cell-sorted particles; false when the binning does not
+// describe the mesh's own cells.  The tile edge is the largest that keeps the
+// LDS footprint within the budget of two (gather: three) workgroups per CU.
+bool make_tiles( TileKind kind, const MeshDev& m, int order, bool fp64,
+                 const int* slot_ng, int nslot,
                  const cabana_amd_binning* bins, const GFieldDev& g0,
-                 TileArgs& t )
+                 TileArgs& t, size_t& lds )
 {
     if ( !bins || !bins->counts || !bins->offsets )
         return false;
@@ -424,15 +490,25 @@ bool make_tiles( const MeshDev& m, int order, bool fp64, int nslot_dofs,
         if ( bins->num_bin[d] != m.num_cell[d] )
             return false;
     const int extra = ( order == 1 ) ? 1 : 3;
-    const size_t real = fp64 ? 8 : 4;
-    const size_t budget = 80 * 1024; // two workgroups per CU
-    int T = 8;
-    while ( T > 2 )
+    const size_t budget = ( kind == TILE_GATHER ) ? 52 * 1024 : 80 * 1024;
+    static const int cand[] = { 8, 6, 4, 3, 2 };
+    int T = 2, cap = 1024;
+    for ( int c : cand )
     {
-        const size_t w = T + extra;
-        if ( w * w * w * nslot_dofs * real + T * T * 12 + 32 <= budget )
+        int cp = 1024;
+        if ( kind == TILE_SCATTER )
+        {
+            // Room for the whole brick at up to 20 particles per cell;
+            // denser bricks run in several batches.
+            cp = 20 * c * c * c;
+            cp = cp < 1024 ? 1024 : ( cp > 8192 ? 8192 : cp );
+        }
+        if ( tile_lds_bytes( kind, fp64, slot_ng, nslot, c, order, cp ) <= budget )
+        {
+            T = c;
+            cap = cp;
             break;
-        T -= 2;
+        }
     }
     std::memset( &t, 0, sizeof( t ) );
     t.cell_offsets = bins->offsets;
@@ -440,15 +516,19 @@ bool make_tiles( const MeshDev& m, int order, bool fp64, int nslot_dofs,
     int ntiles = 1;
     for ( int d = 0; d < 3; ++d )
     {
-        t.tile[d] = T < m.num_cell[d] ? T : m.num_cell[d];
-        t.ntile[d] = ( m.num_cell[d] + t.tile[d] - 1 ) / t.tile[d];
-        t.win[d] = t.tile[d] + extra;
+        t.tile[d] = T;
+        t.ntile[d] = ( m.num_cell[d] + T - 1 ) / T;
+        t.win[d] = T + extra;
         ntiles *= t.ntile[d];
     }
     t.tiles_per_xcd = ( ntiles + 7 ) / 8;
-    t.max_runs = t.tile[0] * t.tile[1];
+    t.max_runs = T * T;
+    t.max_win_nodes = t.win[0] * t.win[1] * t.win[2];
+    const int nbase = T + ( order == 2 ? 1 : 0 );
+    t.max_base_cells = nbase * nbase * nbase;
+    t.cap = cap;
     t.bin_begin = bins->begin;
-    // Flush order: largest grid stride first.
+    // Window walk order: largest grid stride first.
     int perm[3] = { 0, 1, 2 };
     for ( int a = 0; a < 3; ++a )
         for ( int b = a + 1; b < 3; ++b )
@@ -460,57 +540,111 @@ bool make_tiles( const MeshDev& m, int order, bool fp64, int nslot_dofs,
             }
     for ( int d = 0; d < 3; ++d )
         t.flush_perm[d] = perm[d];
+    lds = tile_lds_bytes( kind, fp64, slot_ng, nslot, T, order, cap );
     return true;
 }
 
-// Runs a scatter pass over all particles: tiled over [bins.begin, bins.end)
-// when the binning matches the mesh, atomic elsewhere.
+bool bins_usable( const cabana_amd_binning* bins, int64_t n )
+{
+    return bins && bins->begin >= 0 && bins->begin <= bins->end &&
+           bins->end <= n;
+}
+
+// Per-slot grid dofs of a pass: SHARED plans stage one 3-dof grid.
+int slot_dofs( int plan, const int* forms, int nterms, int* ng )
+{
+    if ( plan == PLAN_MPM_G2P )
+    {
+        ng[0] = 3;
+        return 1;
+    }
+    for ( int i = 0; i < nterms; ++i )
+        ng[i] = form_gdof( forms[i] );
+    return nterms;
+}
+
+// Scatter pass over all particles: cell-register tiles over
+// [bins.begin, bins.end) when the binning matches the mesh, atomics elsewhere.
 cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
                                 int plan, ScatterArgs a, int64_t n,
                                 const cabana_amd_binning* bins,
                                 hipStream_t s )
 {
-    int dofs = 0;
-    if ( plan == PLAN_MPM_G2P )
-        dofs = 3;
-    else
-        for ( int i = 0; i < a.nterms; ++i )
-            dofs += form_gdof( a.t[i].form );
+    int forms[MAX_TERMS], ng[MAX_TERMS];
+    for ( int i = 0; i < a.nterms; ++i )
+        forms[i] = a.t[i].form;
+    const int nslot = slot_dofs( plan, forms, a.nterms, ng );
     TileArgs t;
-    GFieldDev g0 = a.t[0].grid.data ? a.t[0].grid : a.t[0].gx;
+    size_t lds = 0;
     const bool tiled =
-        bins && bins->begin >= 0 && bins->begin <= bins->end &&
-        bins->end <= n && make_tiles( a.mesh, order, fp64, dofs, bins, g0, t );
-    if ( tiled )
-    {
-        if ( bins->begin > 0 )
-        {
-            a.begin = 0;
-            a.end = bins->begin;
-            CA_HIP( ctx, run_scatter_atomic( ctx, order, fp64, plan, a, s ),
-                    "scatter (atomic)" );
-        }
-        if ( bins->end > bins->begin )
-        {
-            a.begin = bins->begin;
-            a.end = bins->end;
-            CA_HIP( ctx, run_scatter_tiled( ctx, order, fp64, plan, a, t, s ),
-                    "scatter (tiled)" );
-        }
-        if ( bins->end < n )
-        {
-            a.begin = bins->end;
-            a.end = n;
-            CA_HIP( ctx, run_scatter_atomic( ctx, order, fp64, plan, a, s ),
-                    "scatter (atomic)" );
-        }
-    }
-    else
+        !a.has_xbar && bins_usable( bins, n ) &&
+        make_tiles( TILE_SCATTER, a.mesh, order, fp64, ng, nslot, bins,
+                    a.t[0].grid, t, lds );
+    if ( !tiled )
     {
         a.begin = 0;
         a.end = n;
         CA_HIP( ctx, run_scatter_atomic( ctx, order, fp64, plan, a, s ),
                 "scatter (atomic)" );
+        return CABANA_AMD_OK;
+    }
+    if ( bins->begin > 0 )
+    {
+        a.begin = 0;
+        a.end = bins->begin;
+        CA_HIP( ctx, run_scatter_atomic( ctx, order, fp64, plan, a, s ),
+                "scatter (atomic)" );
+    }
+    if ( bins->end > bins->begin )
+        CA_HIP( ctx, run_scatter_cells( ctx, order, fp64, plan, a, t, lds, s ),
+                "scatter (cells)" );
+    if ( bins->end < n )
+    {
+        a.begin = bins->end;
+        a.end = n;
+        CA_HIP( ctx, run_scatter_atomic( ctx, order, fp64, plan, a, s ),
+                "scatter (atomic)" );
+    }
+    return CABANA_AMD_OK;
+}
+
+// Gather pass over all particles: LDS-staged tiles over [bins.begin,
+// bins.end) when the binning matches the mesh, direct loads elsewhere.
+cabana_amd_status gather_pass( cabana_amd_context* ctx, int order, bool fp64,
+                               int plan, GatherArgs a, int64_t n,
+                               const cabana_amd_binning* bins, hipStream_t s )
+{
+    int forms[MAX_TERMS], ng[MAX_TERMS];
+    for ( int i = 0; i < a.nterms; ++i )
+        forms[i] = a.t[i].form;
+    const int nslot = slot_dofs( plan, forms, a.nterms, ng );
+    TileArgs t;
+    size_t lds = 0;
+    const bool tiled =
+        bins_usable( bins, n ) &&
+        make_tiles( TILE_GATHER, a.mesh, order, fp64, ng, nslot, bins,
+                    a.t[0].grid, t, lds );
+    if ( !tiled )
+    {
+        a.begin = 0;
+        a.end = n;
+        CA_HIP( ctx, run_gather( ctx, order, fp64, plan, a, s ), "gather" );
+        return CABANA_AMD_OK;
+    }
+    if ( bins->begin > 0 )
+    {
+        a.begin = 0;
+        a.end = bins->begin;
+        CA_HIP( ctx, run_gather( ctx, order, fp64, plan, a, s ), "gather" );
+    }
+    if ( bins->end > bins->begin )
+        CA_HIP( ctx, run_gather_tiled( ctx, order, fp64, plan, a, t, lds, s ),
+                "gather (tiled)" );
+    if ( bins->end < n )
+    {
+        a.begin = bins->end;
+        a.end = n;
+        CA_HIP( ctx, run_gather( ctx, order, fp64, plan, a, s ), "gather" );
     }
     return CABANA_AMD_OK;
 }
@@ -761,29 +895,21 @@ extern "C" cabana_amd_status cabana_amd_g2p(
     }
     if ( n <= 0 )
         return CABANA_AMD_OK;
-    a.begin = 0;
-    a.end = n;
     const bool pair = num_terms == 2 && a.t[0].form == FORM_VV &&
                       a.t[1].form == FORM_TDT &&
                       same_grid( a.t[0].grid, a.t[1].grid );
     if ( num_terms == 1 || pair )
     {
         a.nterms = num_terms;
-        CA_HIP( ctx,
-                run_gather( ctx, order, fp64, pair ? PLAN_MPM_G2P : PLAN_SINGLE,
-                            a, s ),
-                "gather" );
+        return gather_pass( ctx, order, fp64, pair ? PLAN_MPM_G2P : PLAN_SINGLE,
+                            a, n, bins, s );
     }
-    else
+    for ( int i = 0; i < num_terms; ++i )
     {
-        for ( int i = 0; i < num_terms; ++i )
-        {
-            GatherArgs b = a;
-            b.nterms = 1;
-            b.t[0] = a.t[i];
-            CA_HIP( ctx, run_gather( ctx, order, fp64, PLAN_SINGLE, b, s ),
-                    "gather" );
-        }
+        GatherArgs b = a;
+        b.nterms = 1;
+        b.t[0] = a.t[i];
+        CA_CHECK( gather_pass( ctx, order, fp64, PLAN_SINGLE, b, n, bins, s ) );
     }
     return CABANA_AMD_OK;
 }
@@ -855,30 +981,21 @@ extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
     }
     if ( n <= 0 )
         return CABANA_AMD_OK;
-    a.begin = 0;
-    a.end = n;
     const bool mpm = num_terms == 3 && xbar && a.t[0].form == FORM_SV &&
                      a.t[1].form == FORM_VV && a.t[2].form == FORM_TD;
     if ( num_terms == 1 || mpm )
     {
         a.nterms = num_terms;
-        CA_HIP( ctx,
-                run_gather( ctx, order, fp64, mpm ? PLAN_MPM_P2G : PLAN_SINGLE,
-                            a, s ),
-                "p2g adjoint gather" );
+        return gather_pass( ctx, order, fp64, mpm ? PLAN_MPM_P2G : PLAN_SINGLE,
+                            a, n, bins, s );
     }
-    else
+    for ( int i = 0; i < num_terms; ++i )
     {
-        for ( int i = 0; i < num_terms; ++i )
-        {
-            GatherArgs b = a;
-            b.nterms = 1;
-            b.t[0] = a.t[i];
-            CA_HIP( ctx, run_gather( ctx, order, fp64, PLAN_SINGLE, b, s ),
-                    "p2g adjoint gather" );
-        }
+        GatherArgs b = a;
+        b.nterms = 1;
+        b.t[0] = a.t[i];
+        CA_CHECK( gather_pass( ctx, order, fp64, PLAN_SINGLE, b, n, bins, s ) );
     }
-    (void)bins;
     return CABANA_AMD_OK;
 }
 
@@ -958,23 +1075,67 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
             CA_HIP( ctx, launch_zero_ghosts( *ctx, fp64, m, a.t[i].grid, s ),
                     "zero ghosts" );
     }
-    if ( n > 0 )
+    if ( n > 0 && xbar )
     {
-        const bool pair = num_terms == 2 && xbar && a.t[0].form == FORM_VV &&
-                          a.t[1].form == FORM_TDT && a.t[0].grid.data &&
-                          same_grid( a.t[0].grid, a.t[1].grid ) &&
-                          same_grid( a.t[0].gx, a.t[1].gx );
-        if ( num_terms == 1 || pair )
+        // Position cotangents: a G2P-shaped gather of the forward grid,
+        // contracted with the output cotangents (no value output).
+        GatherArgs ga;
+        std::memset( &ga, 0, sizeof( ga ) );
+        ga.mesh = m;
+        ga.skipped = ctx->skipped;
+        ga.points = a.points;
+        ga.xbar = a.xbar;
+        ga.has_xbar = 1;
+        for ( int i = 0; i < num_terms; ++i )
         {
-            a.nterms = num_terms;
-            CA_CHECK( scatter_pass( ctx, order, fp64,
-                                    pair ? PLAN_MPM_G2P : PLAN_SINGLE, a, n,
-                                    bins, s ) );
+            ga.t[i].form = a.t[i].form;
+            ga.t[i].mult = a.t[i].mult;
+            ga.t[i].px = a.t[i].pin;
+            ga.t[i].grid = a.t[i].gx;
+        }
+        const bool gpair = num_terms == 2 && a.t[0].form == FORM_VV &&
+                           a.t[1].form == FORM_TDT &&
+                           same_grid( a.t[0].gx, a.t[1].gx );
+        if ( num_terms == 1 || gpair )
+        {
+            ga.nterms = num_terms;
+            CA_CHECK( gather_pass( ctx, order, fp64,
+                                   gpair ? PLAN_MPM_G2P : PLAN_SINGLE, ga, n,
+                                   bins, s ) );
         }
         else
         {
             for ( int i = 0; i < num_terms; ++i )
             {
+                GatherArgs b = ga;
+                b.nterms = 1;
+                b.t[0] = ga.t[i];
+                CA_CHECK( gather_pass( ctx, order, fp64, PLAN_SINGLE, b, n,
+                                       bins, s ) );
+            }
+        }
+    }
+    // Grid cotangents: a P2G-shaped scatter of the output cotangents.
+    a.has_xbar = 0;
+    if ( n > 0 )
+    {
+        const bool pair = num_terms == 2 && a.t[0].form == FORM_VV &&
+                          a.t[1].form == FORM_TDT && a.t[0].grid.data &&
+                          same_grid( a.t[0].grid, a.t[1].grid );
+        if ( num_terms == 1 || pair )
+        {
+            a.nterms = num_terms;
+            if ( a.t[0].grid.data )
+                CA_CHECK( scatter_pass( ctx, order, fp64,
+                                        pair ? PLAN_MPM_G2P : PLAN_SINGLE, a, n,
+                                        bins, s ) );
+        }
+        else
+        {
+            for ( int i = 0; i < num_terms; ++i )
+            {
+                if ( !a.t[i].grid.data )
+                    continue;
                 ScatterArgs b = a;
                 b.nterms = 1;
                 b.t[0] = a.t[i];

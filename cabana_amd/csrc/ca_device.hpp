@@ -4,6 +4,8 @@
 
 #include "ca_internal.hpp"
 
+#include <utility>
+
 namespace cabana_amd
 {
 
@@ -141,6 +143,27 @@ __device__ __forceinline__ bool eval_spline( const MeshDev& m,
         double x0 = logical_coord<ORDER>( px[d], m.low[d], m.rdx[d] );
         ok = ok && eval_spline_1d<ORDER, Real, NEED_H>(
                        x0, (Real)m.rdx[d], m.num_node[d], s.d[d] );
+    }
+    return ok;
+}
+
+// Stencil base node of a position along every dimension (the first node of
+// Spline<ORDER>::stencil), with the same FP64 rounding as eval_spline.
+// Returns false where eval_spline would.
+template <int ORDER>
+__device__ __forceinline__ bool stencil_base( const MeshDev& m,
+                                              const double px[3], int b[3] )
+{
+    constexpr int K = ORDER + 1;
+    bool ok = true;
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        const double x0 = logical_coord<ORDER>( px[d], m.low[d], m.rdx[d] );
+        const bool fin = x0 >= 0.0 && x0 < (double)m.num_node[d];
+        const int i0 = fin ? (int)x0 : 0;
+        b[d] = ( ORDER >= 2 ) ? i0 - 1 : i0;
+        ok = ok && fin && b[d] >= 0 && b[d] + K <= m.num_node[d];
     }
     return ok;
 }
@@ -358,6 +381,351 @@ __device__ __forceinline__ void form_dx( const Real* P, const Real* G,
         xb[e] += v;
     }
 }
+
+// ---------------------------------------------------------------------------
+// Jets: derivatives of the interpolated grid field at a particle.
+//
+// For one grid component f on the K^3 stencil, the value, gradient and
+// Hessian of sum_n W_n(x) f_n are tensor-product contractions of f with the
+// 1-D weights w, gradients g and second derivatives h.  They are evaluated
+// separably (contract k, then j, then i), which needs far fewer operations
+// than forming each node's weight products.
+// ---------------------------------------------------------------------------
+
+template <class Real>
+struct Jet
+{
+    Real v;
+    Real g[3];
+    Real h[6]; // xx, yy, zz, xy, xz, yz
+};
+
+// J[c] for NG grid components; Load(i, j, k, f) fills f[0..NG) of stencil
+// node (i, j, k).  V, G, H select value, gradient and Hessian.
+template <int ORDER, int NG, bool V, bool G, bool H, class Real, bool SH,
+          class Load>
+__device__ __forceinline__ void eval_jets( const Spline3D<ORDER, Real, SH>& s,
+                                           Load&& load, Jet<Real>* J )
+{
+    constexpr int K = ORDER + 1;
+    constexpr bool NG1 = G || H; // need first derivatives in y/z stages
+#pragma unroll
+    for ( int c = 0; c < NG; ++c )
+    {
+        J[c].v = Real( 0 );
+#pragma unroll
+        for ( int e = 0; e < 3; ++e )
+            J[c].g[e] = Real( 0 );
+#pragma unroll
+        for ( int e = 0; e < 6; ++e )
+            J[c].h[e] = Real( 0 );
+    }
+#pragma unroll
+    for ( int i = 0; i < K; ++i )
+    {
+        Real Uww[NG], Ugw[NG], Uwg[NG], Uhw[NG], Uwh[NG], Ugg[NG];
+#pragma unroll
+        for ( int c = 0; c < NG; ++c )
+            Uww[c] = Ugw[c] = Uwg[c] = Uhw[c] = Uwh[c] = Ugg[c] = Real( 0 );
+#pragma unroll
+        for ( int j = 0; j < K; ++j )
+        {
+            Real Tw[NG], Tg[NG], Th[NG];
+#pragma unroll
+            for ( int c = 0; c < NG; ++c )
+                Tw[c] = Tg[c] = Th[c] = Real( 0 );
+#pragma unroll
+            for ( int k = 0; k < K; ++k )
+            {
+                Real f[NG];
+                load( i, j, k, f );
+                const Real wz = s.d[2].w[k];
+#pragma unroll
+                for ( int c = 0; c < NG; ++c )
+                {
+                    Tw[c] += wz * f[c];
+                    if constexpr ( NG1 )
+                        Tg[c] += s.d[2].g[k] * f[c];
+                    if constexpr ( H )
+                        Th[c] += s.d[2].h[k] * f[c];
+                }
+            }
+            const Real wy = s.d[1].w[j];
+#pragma unroll
+            for ( int c = 0; c < NG; ++c )
+            {
+                Uww[c] += wy * Tw[c];
+                if constexpr ( NG1 )
+                {
+                    Ugw[c] += s.d[1].g[j] * Tw[c];
+                    Uwg[c] += wy * Tg[c];
+                }
+                if constexpr ( H )
+                {
+                    Uhw[c] += s.d[1].h[j] * Tw[c];
+                    Uwh[c] += wy * Th[c];
+                    Ugg[c] += s.d[1].g[j] * Tg[c];
+                }
+            }
+        }
+        const Real wx = s.d[0].w[i], gx = s.d[0].g[i];
+#pragma unroll
+        for ( int c = 0; c < NG; ++c )
+        {
+            if constexpr ( V )
+                J[c].v += wx * Uww[c];
+            if constexpr ( G )
+            {
+                J[c].g[0] += gx * Uww[c];
+                J[c].g[1] += wx * Ugw[c];
+                J[c].g[2] += wx * Uwg[c];
+            }
+            if constexpr ( H )
+            {
+                J[c].h[0] += s.d[0].h[i] * Uww[c];
+                J[c].h[1] += wx * Uhw[c];
+                J[c].h[2] += wx * Uwh[c];
+                J[c].h[3] += gx * Ugw[c];
+                J[c].h[4] += gx * Uwg[c];
+                J[c].h[5] += wx * Ugg[c];
+            }
+        }
+        // Keep the next slab's loads from being hoisted above this one: the
+        // fully unrolled stencil would otherwise hold every node in VGPRs.
+        __builtin_amdgcn_sched_barrier( 0 );
+    }
+}
+
+__device__ __forceinline__ constexpr int hidx( int e, int f )
+{
+    return e == f ? e
+                  : ( ( e < f ? e : f ) == 0 ? ( ( e < f ? f : e ) == 1 ? 3 : 4 )
+                                             : 5 );
+}
+
+// Jet levels a form needs from its grid: value-type forms (SV, VV) read the
+// value, gradient-type forms read the gradient; the position adjoint adds one
+// derivative order.
+__device__ __forceinline__ constexpr bool form_value_type( int form )
+{
+    return form == FORM_SV || form == FORM_VV;
+}
+
+// dB/dP of form FORM from the jets of its grid (the G2P output / the P2G
+// value cotangent before the multiplier).
+template <int FORM, class Real>
+__device__ __forceinline__ void jet_dP( const Jet<Real>* J, Real* out )
+{
+    if constexpr ( FORM == FORM_SV )
+        out[0] = J[0].v;
+    else if constexpr ( FORM == FORM_VV )
+    {
+#pragma unroll
+        for ( int d = 0; d < 3; ++d )
+            out[d] = J[d].v;
+    }
+    else if constexpr ( FORM == FORM_SG )
+        out[0] = J[0].g[0] + J[1].g[1] + J[2].g[2];
+    else if constexpr ( FORM == FORM_VD )
+    {
+#pragma unroll
+        for ( int d = 0; d < 3; ++d )
+            out[d] = J[0].g[d];
+    }
+    else if constexpr ( FORM == FORM_TD )
+    {
+#pragma unroll
+        for ( int a = 0; a < 3; ++a )
+#pragma unroll
+            for ( int b = 0; b < 3; ++b )
+                out[3 * a + b] = J[a].g[b];
+    }
+    else
+    {
+#pragma unroll
+        for ( int a = 0; a < 3; ++a )
+#pragma unroll
+            for ( int b = 0; b < 3; ++b )
+                out[3 * a + b] = J[b].g[a];
+    }
+}
+
+// dB/dx_e of form FORM for particle vector P from the jets of its grid.
+template <int FORM, class Real>
+__device__ __forceinline__ void jet_dx( const Real* P, const Jet<Real>* J,
+                                        Real* xb )
+{
+#pragma unroll
+    for ( int e = 0; e < 3; ++e )
+    {
+        Real v;
+        if constexpr ( FORM == FORM_SV )
+            v = P[0] * J[0].g[e];
+        else if constexpr ( FORM == FORM_VV )
+            v = P[0] * J[0].g[e] + P[1] * J[1].g[e] + P[2] * J[2].g[e];
+        else if constexpr ( FORM == FORM_SG )
+            v = P[0] * ( J[0].h[hidx( e, 0 )] + J[1].h[hidx( e, 1 )] +
+                         J[2].h[hidx( e, 2 )] );
+        else if constexpr ( FORM == FORM_VD )
+            v = P[0] * J[0].h[hidx( e, 0 )] + P[1] * J[0].h[hidx( e, 1 )] +
+                P[2] * J[0].h[hidx( e, 2 )];
+        else if constexpr ( FORM == FORM_TD )
+        {
+            v = Real( 0 );
+#pragma unroll
+            for ( int a = 0; a < 3; ++a )
+#pragma unroll
+                for ( int b = 0; b < 3; ++b )
+                    v += P[3 * a + b] * J[a].h[hidx( e, b )];
+        }
+        else
+        {
+            v = Real( 0 );
+#pragma unroll
+            for ( int a = 0; a < 3; ++a )
+#pragma unroll
+                for ( int b = 0; b < 3; ++b )
+                    v += P[3 * a + b] * J[b].h[hidx( e, a )];
+        }
+        xb[e] += v;
+    }
+}
+
+template <class F, std::size_t... I>
+__device__ __forceinline__ void static_for_impl( F&& f,
+                                                 std::index_sequence<I...> )
+{
+    ( f( std::integral_constant<int, (int)I>{} ), ... );
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for( F&& f )
+{
+    static_for_impl( f, std::make_index_sequence<N>{} );
+}
+
+// Jet levels every grid slot of a gather pass needs (SHARED: one slot read
+// by every term).
+template <int... FORMS>
+struct GatherNeeds
+{
+    static constexpr int N = sizeof...( FORMS );
+    static constexpr int f[N] = { FORMS... };
+    static constexpr bool uses( int t, int slot, bool shared )
+    {
+        return shared || t == slot;
+    }
+    static constexpr bool V( int slot, bool shared )
+    {
+        for ( int t = 0; t < N; ++t )
+            if ( uses( t, slot, shared ) && form_value_type( f[t] ) )
+                return true;
+        return false;
+    }
+    static constexpr bool G( int slot, bool shared, bool xbar )
+    {
+        for ( int t = 0; t < N; ++t )
+            if ( uses( t, slot, shared ) &&
+                 ( !form_value_type( f[t] ) || xbar ) )
+                return true;
+        return false;
+    }
+    static constexpr bool H( int slot, bool shared, bool xbar )
+    {
+        for ( int t = 0; t < N; ++t )
+            if ( uses( t, slot, shared ) && !form_value_type( f[t] ) && xbar )
+                return true;
+        return false;
+    }
+    static constexpr int NG( int slot, bool shared )
+    {
+        return shared ? 3
+                      : ( ( f[slot] == FORM_SV || f[slot] == FORM_VD ) ? 1 : 3 );
+    }
+};
+
+// One particle of a gather pass (G2P forward, P2G adjoint, or the position
+// part of the G2P adjoint): jets of every grid slot, then for each term
+//   pout += mult * dB/dP      and      xbar += mult * dB/dx.
+// Load(slot_constant, i, j, k, f) fills the slot's NG values of stencil node
+// (i, j, k).
+template <int ORDER, class Real, bool XBAR, bool SHARED, int... FORMS,
+          class Load>
+__device__ __forceinline__ void gather_particle(
+    const GatherArgs& a, int64_t p, const Spline3D<ORDER, Real, XBAR>& s,
+    Load&& load )
+{
+    using GN = GatherNeeds<FORMS...>;
+    constexpr int NT = GN::N;
+    constexpr int NSLOT = SHARED ? 1 : NT;
+    Jet<Real> J[NSLOT][3];
+    static_for<NSLOT>( [&]( auto S ) {
+        constexpr int sl = S;
+        constexpr int NG = GN::NG( sl, SHARED );
+        eval_jets<ORDER, NG, GN::V( sl, SHARED ), GN::G( sl, SHARED, XBAR ),
+                  GN::H( sl, SHARED, XBAR )>(
+            s, [&]( int i, int j, int k, Real* fv ) { load( S, i, j, k, fv ); },
+            J[sl] );
+    } );
+    double xb[3] = { 0.0, 0.0, 0.0 };
+    static_for<NT>( [&]( auto I ) {
+        constexpr int t = I;
+        constexpr int F = GN::f[t];
+        constexpr int NP = FormTraits<F>::NP;
+        const Jet<Real>* Jt = J[SHARED ? 0 : t];
+        const Real m = (Real)a.t[t].mult;
+        const PFieldDev& o = a.t[t].pout;
+        if ( o.data )
+        {
+            Real out[NP];
+            jet_dP<F>( Jt, out );
+            const int64_t b = o.base( p );
+#pragma unroll
+            for ( int c = 0; c < NP; ++c )
+                *o.template ptr<Real>( b, c ) += m * out[c];
+        }
+        if constexpr ( XBAR )
+        {
+            Real P[NP];
+            const int64_t b = a.t[t].px.base( p );
+#pragma unroll
+            for ( int c = 0; c < NP; ++c )
+                P[c] = *a.t[t].px.template ptr<const Real>( b, c );
+            Real xt[3] = { Real( 0 ), Real( 0 ), Real( 0 ) };
+            jet_dx<F>( P, Jt, xt );
+#pragma unroll
+            for ( int e = 0; e < 3; ++e )
+                xb[e] += (double)( m * xt[e] );
+        }
+    } );
+    if constexpr ( XBAR )
+    {
+        const int64_t b = a.xbar.base( p );
+#pragma unroll
+        for ( int e = 0; e < 3; ++e )
+            *a.xbar.ptr<double>( b, e ) += xb[e];
+    }
+}
+
+// Loader of stencil nodes straight from the grid arrays.
+template <class Real, bool SHARED, int... FORMS>
+struct GlobalGatherLoad
+{
+    const GatherArgs& a;
+    int base[3];
+    template <class IC>
+    __device__ __forceinline__ void operator()( IC, int i, int j, int k,
+                                                Real* f ) const
+    {
+        constexpr int sl = IC::value;
+        constexpr int NG = GatherNeeds<FORMS...>::NG( sl, SHARED );
+        const GFieldDev& g = a.t[sl].grid;
+        const Real* q = gptr<const Real>( g, ( base[0] + i ) * g.s[0] + ( base[1] + j ) * g.s[1] +
+                        ( base[2] + k ) * g.s[2] );
+#pragma unroll
+        for ( int d = 0; d < NG; ++d )
+            f[d] = q[d * g.s[3]];
+    }
+};
 
 template <class Real>
 __device__ __forceinline__ void atomic_add( Real* p, Real v )

@@ -64,8 +64,8 @@ __global__ __launch_bounds__( BLOCK ) void halo_fill_kernel( MeshDev m,
     const int ok = owner_of( m, 2, k, w, gk );
     if ( !gi && !gj && !gk )
         return; // owned
-    Real* dst = reinterpret_cast<Real*>( g.data ) + i * g.s[0] + j * g.s[1] +
-                k * g.s[2];
+    Real* dst = gptr<Real>( g, i * g.s[0] + j * g.s[1] +
+                k * g.s[2] );
     if ( oi < 0 || oj < 0 || ok < 0 )
     {
         if ( zero_other )
@@ -73,8 +73,8 @@ __global__ __launch_bounds__( BLOCK ) void halo_fill_kernel( MeshDev m,
                 dst[d * g.s[3]] = Real( 0 );
         return;
     }
-    const Real* src = reinterpret_cast<const Real*>( g.data ) + oi * g.s[0] +
-                      oj * g.s[1] + ok * g.s[2];
+    const Real* src = gptr<const Real>( g, oi * g.s[0] +
+                      oj * g.s[1] + ok * g.s[2] );
     for ( int d = 0; d < g.ext[3]; ++d )
         dst[d * g.s[3]] = src[d * g.s[3]];
 }
@@ -120,8 +120,8 @@ __global__ __launch_bounds__( BLOCK ) void halo_fold_kernel( MeshDev m,
     }
     if ( !any )
         return;
-    Real* dst = reinterpret_cast<Real*>( g.data ) + x[0] * g.s[0] +
-                x[1] * g.s[1] + x[2] * g.s[2];
+    Real* dst = gptr<Real>( g, x[0] * g.s[0] +
+                x[1] * g.s[1] + x[2] * g.s[2] );
     for ( int d = 0; d < g.ext[3]; ++d )
     {
         Real acc = Real( 0 );
@@ -131,9 +131,9 @@ __global__ __launch_bounds__( BLOCK ) void halo_fold_kernel( MeshDev m,
                 {
                     if ( a == 0 && b == 0 && c == 0 )
                         continue;
-                    acc += reinterpret_cast<const Real*>(
-                        g.data )[cand[0][a] * g.s[0] + cand[1][b] * g.s[1] +
-                                 cand[2][c] * g.s[2] + d * g.s[3]];
+                    acc += *gptr<const Real>(
+                        g, cand[0][a] * g.s[0] + cand[1][b] * g.s[1] +
+                               cand[2][c] * g.s[2] + d * g.s[3] );
                 }
         dst[d * g.s[3]] += acc;
     }
@@ -156,8 +156,8 @@ __global__ __launch_bounds__( BLOCK ) void zero_ghosts_kernel( MeshDev m,
         owned = owned && x[d] >= m.halo && x[d] < owned_hi( m, d );
     if ( owned )
         return;
-    Real* dst = reinterpret_cast<Real*>( g.data ) + x[0] * g.s[0] +
-                x[1] * g.s[1] + x[2] * g.s[2];
+    Real* dst = gptr<Real>( g, x[0] * g.s[0] +
+                x[1] * g.s[1] + x[2] * g.s[2] );
     for ( int d = 0; d < g.ext[3]; ++d )
         dst[d * g.s[3]] = Real( 0 );
 }

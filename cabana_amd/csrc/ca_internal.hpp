@@ -50,12 +50,22 @@ struct PFieldDev
     }
 };
 
+// Grid array: element strides are 32-bit and every element's byte offset fits
+// in 32 bits (checked on the host), so device addressing is a uniform 64-bit
+// base plus a 32-bit offset.
 struct GFieldDev
 {
     char* data;
-    int64_t s[4];
+    int32_t s[4];
     int32_t ext[4];
 };
+
+template <class T>
+__device__ __forceinline__ T* gptr( const GFieldDev& g, int elem )
+{
+    return reinterpret_cast<T*>(
+        g.data + (uint32_t)( (uint32_t)elem * (uint32_t)sizeof( T ) ) );
+}
 
 // Bilinear forms B(P, G; W, dW) between a particle vector P and a node vector
 // G.  P2G scatters dB/dG, G2P gathers dB/dP and the position adjoint
@@ -160,6 +170,9 @@ struct TileArgs
     int32_t tiles_per_xcd;       // blocks are dealt round-robin to 8 XCDs
     int32_t flush_perm[3];       // grid dims from largest to smallest stride
     int32_t max_runs;            // tile[0] * tile[1]
+    int32_t max_win_nodes;       // win[0] * win[1] * win[2]
+    int32_t max_base_cells;      // stencil-base cells of an unclipped tile
+    int32_t cap;                 // particles sorted per batch (LDS capacity)
     int64_t bin_begin;           // particle index of cell offset 0
 };
 
@@ -189,11 +202,19 @@ enum Plan : int32_t
                                        const GatherArgs& a, hipStream_t s );   \
     hipError_t launch_scatter_atomic_##SUFFIX( Launcher& L, bool fp64,         \
                                                int plan, const ScatterArgs& a, \
-                                               hipStream_t s );                \
-    hipError_t launch_scatter_tiled_##SUFFIX( Launcher& L, bool fp64,          \
+                                               hipStream_t s );
+#define CABANA_AMD_DECLARE_TILED( SUFFIX )                                     \
+    hipError_t launch_gather_tiled_##SUFFIX( Launcher& L, bool fp64, int plan, \
+                                             const GatherArgs& a,              \
+                                             const TileArgs& t, size_t lds,    \
+                                             hipStream_t s );                  \
+    hipError_t launch_scatter_cells_##SUFFIX( Launcher& L, bool fp64,          \
                                               int plan, const ScatterArgs& a,  \
-                                              const TileArgs& t,               \
+                                              const TileArgs& t, size_t lds,   \
                                               hipStream_t s );
+CABANA_AMD_DECLARE_TILED( o1 )
+CABANA_AMD_DECLARE_TILED( o2 )
+CABANA_AMD_DECLARE_TILED( o3 )
 CABANA_AMD_DECLARE_LAUNCHERS( o1 )
 CABANA_AMD_DECLARE_LAUNCHERS( o2 )
 CABANA_AMD_DECLARE_LAUNCHERS( o3 )

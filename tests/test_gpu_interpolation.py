@@ -122,11 +122,26 @@ def test_interpolation3d_known_answers(ca, oracle, order, dtype):
     assert ca.check_domain() == 0
 
 
+def sort_case(ca, c):
+    """Cell-sort the case's points with the library's LinkedCellList;
+    returns the binning and the permutation applied."""
+    gm = c.lg.globalGrid().globalMesh()
+    lcl = ca.createLinkedCellList(c.pts, [gm.cellSize(d) for d in range(3)],
+                                  [gm.lowCorner(d) for d in range(3)],
+                                  [gm.highCorner(d) for d in range(3)])
+    perm = lcl.permutes.cpu().numpy()
+    c.pts = c.pts[lcl.permutes].contiguous()
+    c.pts_np = c.pts_np[perm]
+    return lcl, perm
+
+
+@pytest.mark.parametrize("sorted_bins", [False, True])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("order", [1, 2, 3])
 @pytest.mark.parametrize("op", range(5))
-def test_p2g_matches_oracle(ca, oracle, op, order, dtype):
+def test_p2g_matches_oracle(ca, oracle, op, order, dtype, sorted_bins):
     c = Case(ca, oracle, seed=op * 7 + order)
+    bins = sort_case(ca, c)[0] if sorted_bins else None
     nc, nd = P2G_SHAPE[op]
     vals = c.rng.normal(size=(c.n, nc))
     grid = c.array(nd, dtype)
@@ -135,7 +150,7 @@ def test_p2g_matches_oracle(ca, oracle, op, order, dtype):
     grid.view().copy_(torch.tensor(init, dtype=dtype))
     f = getattr(ca, P2G_FACTORY[op])(shaped(torch.tensor(vals, dtype=dtype, device="cuda"),
                                             nc), -0.6)
-    ca.p2g(f, c.pts, c.n, ca.Spline(order), c.halo(grid), grid)
+    ca.p2g(f, c.pts, c.n, ca.Spline(order), c.halo(grid), grid, bins=bins)
     ref = init.copy()
     oracle.p2g(c.mesh, order, op, -0.6, c.pts_np, vals, ref)
     own = c.mesh.owned_slices()
@@ -144,11 +159,13 @@ def test_p2g_matches_oracle(ca, oracle, op, order, dtype):
     assert ca.check_domain() == 0
 
 
+@pytest.mark.parametrize("sorted_bins", [False, True])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("order", [1, 2, 3])
 @pytest.mark.parametrize("op", range(5))
-def test_g2p_matches_oracle(ca, oracle, op, order, dtype):
+def test_g2p_matches_oracle(ca, oracle, op, order, dtype, sorted_bins):
     c = Case(ca, oracle, seed=100 + op * 7 + order)
+    bins = sort_case(ca, c)[0] if sorted_bins else None
     nc, nd = G2P_SHAPE[op]
     grid = c.array(nd, dtype)
     g = c.rng.normal(size=grid.view().shape)
@@ -156,7 +173,7 @@ def test_g2p_matches_oracle(ca, oracle, op, order, dtype):
     base = c.rng.normal(size=(c.n, nc))
     out = shaped(torch.tensor(base, dtype=dtype, device="cuda"), nc)
     f = getattr(ca, G2P_FACTORY[op])(out, 1.7)
-    ca.g2p(grid, c.halo(grid), c.pts, c.n, ca.Spline(order), f)
+    ca.g2p(grid, c.halo(grid), c.pts, c.n, ca.Spline(order), f, bins=bins)
     ref = base.copy()
     gref = g.copy()
     oracle.g2p(c.mesh, order, op, 1.7, gref, c.pts_np, ref)
@@ -310,3 +327,31 @@ def test_invalid_arguments_raise(ca, oracle):
     with pytest.raises(CabanaAmdError):   # FP32 values into an FP64 grid
         ca.p2g(ca.createVectorValueP2G(f32, 1.0), c.pts, c.n, ca.Spline(2), c.halo(grid),
                grid)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_dense_cells_multiple_batches(ca, oracle, order, dtype):
+    """~200 particles per cell in a cluster: bricks exceed the per-batch sort
+    capacity and base cells are flushed in several batches."""
+    c = Case(ca, oracle, num_cell=(6, 6, 6), cell=0.5, n=1, seed=17)
+    rng = np.random.default_rng(17)
+    gm = c.lg.globalGrid().globalMesh()
+    lo = np.array([gm.lowCorner(d) for d in range(3)])
+    n = 60000
+    centre = lo + 1.4
+    pts = np.clip(centre + rng.normal(scale=0.35, size=(n, 3)), lo, lo + 2.999)
+    c.pts_np = pts
+    c.pts = torch.tensor(pts, device="cuda")
+    c.n = n
+    bins, perm = sort_case(ca, c)
+    vals = rng.normal(size=(n, 9))
+    grid = c.array(3, dtype)
+    f = ca.createTensorDivergenceP2G(torch.tensor(vals.reshape(-1, 3, 3), dtype=dtype,
+                                                  device="cuda"), 0.5)
+    ca.p2g(f, c.pts, n, ca.Spline(order), c.halo(grid), grid, bins=bins)
+    ref = c.mesh.grid(3)
+    oracle.p2g(c.mesh, order, 4, 0.5, c.pts_np, vals, ref)
+    own = c.mesh.owned_slices()
+    assert rel_l2(grid.view().cpu().numpy()[own], ref[own]) <= TOL[dtype]
+    assert ca.check_domain() == 0
