@@ -42,20 +42,27 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--mode", choices=["both", "fp32", "fp64"], default="both")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one elementwise copy (y = 2*x, FP32) of this many MiB: "
+                         "a kernel of known traffic against which rocprofv3 FETCH_SIZE and "
+                         "WRITE_SIZE are calibrated")
     return ap.parse_args()
 
 
 def algorithmic_bytes(kernel, n, s, nodes):
-    """Minimum HBM bytes of one launch: every particle input read once, every
-    accumulated (+=) output read and written once, positions FP64 (24 B),
-    position cotangents FP64 (RMW, 48 B), grid arrays read (or RMW) once."""
-    if kernel.startswith("p2g_mpm"):
+    """Minimum HBM bytes of one launch of a step kernel: every particle input
+    read once (positions FP64, 24 B), every accumulated (+=) particle output
+    read and written once (position cotangents FP64, 48 B), grid arrays read
+    once (gathers) or read and written once (scatters); s = attribute bytes."""
+    if kernel.startswith("p2g_mpm"):            # m, q, sigma -> 7 grid dofs
         return n * (24 + 13 * s) + nodes * 7 * s * 2
-    if kernel.startswith("g2p_mpm"):
+    if kernel.startswith("g2p_mpm"):            # u -> v += , L +=
         return n * (24 + 2 * 12 * s) + nodes * 3 * s
-    if kernel.startswith("g2p_adjoint_mpm"):
-        return n * (24 + 12 * s + 48) + nodes * 3 * s + nodes * 3 * s * 2
-    if kernel.startswith("p2g_adjoint_mpm"):
+    if kernel.startswith("gather_xbar"):        # vbar, Lbar, u -> xbar +=
+        return n * (24 + 12 * s + 48) + nodes * 3 * s
+    if kernel.startswith("g2p_adjoint_mpm"):    # vbar, Lbar -> ubar +=
+        return n * (24 + 12 * s) + nodes * 3 * s * 2
+    if kernel.startswith("p2g_adjoint_mpm"):    # gbar, m, q, sigma -> bars +=, xbar +=
         return n * (24 + 13 * s + 2 * 13 * s + 48) + nodes * 7 * s
     return None
 
@@ -186,6 +193,24 @@ def kernel_profile(native, torch, fn, steps):
     return out
 
 
+def measured_traffic(kernel, workload_tag):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary for this workload (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    copy kernel of known size), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_%s.md" % workload_tag)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 5 and cells[0] == kernel:
+                try:
+                    return float(cells[4])
+                except ValueError:
+                    return None
+    return None
+
+
 def cpu_baseline(cells, ppc, order, budget_s):
     """The oracle (OpenMP restatement of Cabana's p2g/g2p with ScatterView
     duplication, plus OpenMP adjoints) on the host cores, on a bounded sample
@@ -227,10 +252,13 @@ def cpu_baseline(cells, ppc, order, budget_s):
         oracle.p2g_adjoint(mesh, order, 4, -1.0, pts, sig, gbs, sb, xbar, threads=threads)
         return time.perf_counter() - t
 
-    n0 = 1 << 17
-    t0 = run(n0)
-    n = int(min(cells ** 3 * ppc, max(n0, n0 * budget_s / max(t0, 1e-3))))
+    # The per-thread grid copies make small samples cost more per particle:
+    # grow the sample until it takes at least half the budget.
+    n = 1 << 18
     t = run(n)
+    while t < 0.5 * budget_s and n < cells ** 3 * ppc:
+        n = int(min(cells ** 3 * ppc, n * max(2.0, 0.8 * budget_s / max(t, 1e-3))))
+        t = run(n)
     return {"value": n / t, "unit": "particles/s", "cores": threads, "kind": "port",
             "sample": "%d of %d particles, full 128^3 grid, FP64, same fwd+adj operator set "
                       "(oracle: C restatement, OpenMP with per-thread grid copies), %.1f s"
@@ -248,6 +276,12 @@ def main():
     import cabana_amd as ca
     from cabana_amd import _native
     torch.cuda.set_device(0)
+    if args.copy_probe_mib > 0:
+        x = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.mul(x, 2.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
     res = {}
     modes = [("fp32", torch.float32), ("fp64", torch.float64)]
     if args.mode != "both":
@@ -255,11 +289,12 @@ def main():
     for label, dtype in modes:
         w = Workload(torch, ca, dtype, args.cells, args.ppc, args.order)
         ms = time_steps(torch, w.step, args.steps, args.warmup)
-        prof = kernel_profile(_native, torch, w.step, max(2, min(args.steps, 5)))
+        psteps = max(2, min(args.steps, 5))
+        prof = kernel_profile(_native, torch, w.step, psteps)
         ms_sort = time_steps(torch, w.sort_step, 2, 1)
         ca.check_domain()
-        res[label] = {"ms": ms, "rate": w.n / (ms * 1e-3), "kernels": prof, "n": w.n,
-                      "sort_ms": ms_sort}
+        res[label] = {"ms": ms, "rate": w.n / (ms * 1e-3), "kernels": prof,
+                      "prof_steps": psteps, "n": w.n, "sort_ms": ms_sort}
         print("[bench] %s: %.3f ms/step, %.3e particles/s, sort %.3f ms" %
               (label, ms, w.n / (ms * 1e-3), ms_sort), file=sys.stderr, flush=True)
         del w
@@ -273,8 +308,14 @@ def main():
     ab = algorithmic_bytes(dom, head["n"], s, NODES)
     achieved = ab / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic(dom, "%dc_%dppc_o%d_%s" % (
+                    args.cells, args.ppc, args.order, "fp32" if "fp32" in res else "fp64")),
                 "algorithmic_bytes": ab}
+    step_bytes = sum(algorithmic_bytes(k, head["n"], s, NODES) * v["launches"]
+                     for k, v in kernels.items() if algorithmic_bytes(k, 1, s, 1)) / head["prof_steps"]
+    roofline["step_algorithmic_bytes"] = step_bytes
+    roofline["step_frac"] = step_bytes / (head["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cells, args.ppc, args.order, args.cpu_seconds)
