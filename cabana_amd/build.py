@@ -77,6 +77,20 @@ def build_library(verbose=False):
     return LIB
 
 
+def build_cpp_tests():
+    """Host-only C++ programs over include/cabana_amd_grid.hpp (g++, linked
+    against the library and the HIP runtime); run by the GPU tests."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    src = os.path.join(ROOT, "tests", "cpp", "test_grid_shim.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "test_grid_shim")
+    _run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+          "-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"),
+          src, "-o", out, "-L", HERE, "-lcabana_amd", "-L", os.path.join(rocm, "lib"),
+          "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../cabana_amd",
+          "-Wl,-rpath," + os.path.join(rocm, "lib")])
+    return out
+
+
 def build_oracle():
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-C", odir, "-s"])
@@ -87,6 +101,7 @@ def main():
     lib = build_library(verbose="-v" in sys.argv)
     print("built", lib)
     print("built", build_oracle())
+    print("built", build_cpp_tests())
 
 
 if __name__ == "__main__":

@@ -766,9 +766,9 @@ extern "C" cabana_amd_status cabana_amd_p2g(
     MeshDev m;
     CA_CHECK( make_mesh( ctx, mesh, m ) );
     CA_CHECK( check_order( ctx, order ) );
-    if ( !points || num_terms < 1 || num_terms > MAX_TERMS || !terms )
+    if ( !points || num_terms < 1 || num_terms > MAX_CALL_TERMS || !terms )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
-                     "need points and 1..3 terms" );
+                     "need points and 1..16 terms" );
     const int w = resolve_width( m, halo_width );
     CA_CHECK( check_width( ctx, m, w ) );
     const int64_t n = points->num_particle;
@@ -779,6 +779,7 @@ extern "C" cabana_amd_status cabana_amd_p2g(
 
     ScatterArgs a;
     std::memset( &a, 0, sizeof( a ) );
+    std::vector<ScatterTermDev> T( num_terms );
     a.mesh = m;
     a.skipped = ctx->skipped;
     CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, n, false,
@@ -790,27 +791,29 @@ extern "C" cabana_amd_status cabana_amd_p2g(
             return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                          "unknown P2G operator" );
         const int form = p2g_form( t.op );
-        a.t[i].form = form;
-        a.t[i].mult = t.multiplier;
+        T[i].form = form;
+        T[i].mult = t.multiplier;
         CA_CHECK( make_pfield( ctx, t.values, form_pcomp( form ), rdt, n,
-                               false, "values", a.t[i].pin ) );
+                               false, "values", T[i].pin ) );
         CA_CHECK( make_gfield( ctx, m, t.grid, form_gdof( form ), rdt, false,
-                               "grid", a.t[i].grid ) );
+                               "grid", T[i].grid ) );
     }
     for ( int i = 0; i < num_terms; ++i )
         for ( int j = 0; j < i; ++j )
-            if ( a.t[i].grid.data == a.t[j].grid.data &&
-                 !same_grid( a.t[i].grid, a.t[j].grid ) )
+            if ( T[i].grid.data == T[j].grid.data &&
+                 !same_grid( T[i].grid, T[j].grid ) )
                 return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                              "terms share grid memory with different layouts" );
     hipStream_t s = static_cast<hipStream_t>( stream );
     if ( n > 0 )
     {
-        const bool mpm = num_terms == 3 && a.t[0].form == FORM_SV &&
-                         a.t[1].form == FORM_VV && a.t[2].form == FORM_TD;
+        const bool mpm = num_terms == 3 && T[0].form == FORM_SV &&
+                         T[1].form == FORM_VV && T[2].form == FORM_TD;
         if ( num_terms == 1 || mpm )
         {
             a.nterms = num_terms;
+            for ( int i = 0; i < num_terms; ++i )
+                a.t[i] = T[i];
             CA_CHECK( scatter_pass( ctx, order, fp64,
                                     mpm ? PLAN_MPM_P2G : PLAN_SINGLE, a, n,
                                     bins, s ) );
@@ -821,7 +824,7 @@ extern "C" cabana_amd_status cabana_amd_p2g(
             {
                 ScatterArgs b = a;
                 b.nterms = 1;
-                b.t[0] = a.t[i];
+                b.t[0] = T[i];
                 CA_CHECK( scatter_pass( ctx, order, fp64, PLAN_SINGLE, b, n,
                                         bins, s ) );
             }
@@ -832,10 +835,10 @@ extern "C" cabana_amd_status cabana_amd_p2g(
     {
         bool seen = false;
         for ( int j = 0; j < i; ++j )
-            seen = seen || a.t[j].grid.data == a.t[i].grid.data;
+            seen = seen || T[j].grid.data == T[i].grid.data;
         if ( !seen )
             CA_HIP( ctx,
-                    launch_halo_scatter_sum( *ctx, fp64, m, a.t[i].grid, w, s ),
+                    launch_halo_scatter_sum( *ctx, fp64, m, T[i].grid, w, s ),
                     "halo scatter" );
     }
     return CABANA_AMD_OK;
@@ -852,9 +855,9 @@ extern "C" cabana_amd_status cabana_amd_g2p(
     MeshDev m;
     CA_CHECK( make_mesh( ctx, mesh, m ) );
     CA_CHECK( check_order( ctx, order ) );
-    if ( !points || num_terms < 1 || num_terms > MAX_TERMS || !terms )
+    if ( !points || num_terms < 1 || num_terms > MAX_CALL_TERMS || !terms )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
-                     "need points and 1..3 terms" );
+                     "need points and 1..16 terms" );
     const int w = resolve_width( m, halo_width );
     CA_CHECK( check_width( ctx, m, w ) );
     const int64_t n = points->num_particle;
@@ -865,6 +868,7 @@ extern "C" cabana_amd_status cabana_amd_g2p(
 
     GatherArgs a;
     std::memset( &a, 0, sizeof( a ) );
+    std::vector<GatherTermDev> T( num_terms );
     a.mesh = m;
     a.skipped = ctx->skipped;
     CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, n, false,
@@ -876,31 +880,33 @@ extern "C" cabana_amd_status cabana_amd_g2p(
             return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                          "unknown G2P operator" );
         const int form = g2p_form( t.op );
-        a.t[i].form = form;
-        a.t[i].mult = t.multiplier;
+        T[i].form = form;
+        T[i].mult = t.multiplier;
         CA_CHECK( make_pfield( ctx, t.values, form_pcomp( form ), rdt, n,
-                               false, "values", a.t[i].pout ) );
+                               false, "values", T[i].pout ) );
         CA_CHECK( make_gfield( ctx, m, t.grid, form_gdof( form ), rdt, false,
-                               "grid", a.t[i].grid ) );
+                               "grid", T[i].grid ) );
     }
     hipStream_t s = static_cast<hipStream_t>( stream );
     for ( int i = 0; i < num_terms; ++i )
     {
         bool seen = false;
         for ( int j = 0; j < i; ++j )
-            seen = seen || a.t[j].grid.data == a.t[i].grid.data;
+            seen = seen || T[j].grid.data == T[i].grid.data;
         if ( !seen )
-            CA_HIP( ctx, launch_halo_gather( *ctx, fp64, m, a.t[i].grid, w, s ),
+            CA_HIP( ctx, launch_halo_gather( *ctx, fp64, m, T[i].grid, w, s ),
                     "halo gather" );
     }
     if ( n <= 0 )
         return CABANA_AMD_OK;
-    const bool pair = num_terms == 2 && a.t[0].form == FORM_VV &&
-                      a.t[1].form == FORM_TDT &&
-                      same_grid( a.t[0].grid, a.t[1].grid );
+    const bool pair = num_terms == 2 && T[0].form == FORM_VV &&
+                      T[1].form == FORM_TDT &&
+                      same_grid( T[0].grid, T[1].grid );
     if ( num_terms == 1 || pair )
     {
         a.nterms = num_terms;
+        for ( int i = 0; i < num_terms; ++i )
+            a.t[i] = T[i];
         return gather_pass( ctx, order, fp64, pair ? PLAN_MPM_G2P : PLAN_SINGLE,
                             a, n, bins, s );
     }
@@ -908,7 +914,7 @@ extern "C" cabana_amd_status cabana_amd_g2p(
     {
         GatherArgs b = a;
         b.nterms = 1;
-        b.t[0] = a.t[i];
+        b.t[0] = T[i];
         CA_CHECK( gather_pass( ctx, order, fp64, PLAN_SINGLE, b, n, bins, s ) );
     }
     return CABANA_AMD_OK;
@@ -930,9 +936,9 @@ extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
     MeshDev m;
     CA_CHECK( make_mesh( ctx, mesh, m ) );
     CA_CHECK( check_order( ctx, order ) );
-    if ( !points || num_terms < 1 || num_terms > MAX_TERMS || !terms )
+    if ( !points || num_terms < 1 || num_terms > MAX_CALL_TERMS || !terms )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
-                     "need points and 1..3 terms" );
+                     "need points and 1..16 terms" );
     const int w = resolve_width( m, halo_width );
     CA_CHECK( check_width( ctx, m, w ) );
     const int64_t n = points->num_particle;
@@ -944,6 +950,7 @@ extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
 
     GatherArgs a;
     std::memset( &a, 0, sizeof( a ) );
+    std::vector<GatherTermDev> T( num_terms );
     a.mesh = m;
     a.skipped = ctx->skipped;
     CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, n, false,
@@ -959,33 +966,35 @@ extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
             return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                          "unknown P2G operator" );
         const int form = p2g_form( t.op );
-        a.t[i].form = form;
-        a.t[i].mult = t.multiplier;
+        T[i].form = form;
+        T[i].mult = t.multiplier;
         CA_CHECK( make_pfield( ctx, t.values_bar, form_pcomp( form ), rdt, n,
-                               true, "values_bar", a.t[i].pout ) );
+                               true, "values_bar", T[i].pout ) );
         CA_CHECK( make_pfield( ctx, t.values, form_pcomp( form ), rdt, n,
-                               !xbar, "values", a.t[i].px ) );
+                               !xbar, "values", T[i].px ) );
         CA_CHECK( make_gfield( ctx, m, t.grid_bar, form_gdof( form ), rdt,
-                               false, "grid_bar", a.t[i].grid ) );
+                               false, "grid_bar", T[i].grid ) );
     }
     hipStream_t s = static_cast<hipStream_t>( stream );
     for ( int i = 0; i < num_terms; ++i )
     {
         bool seen = false;
         for ( int j = 0; j < i; ++j )
-            seen = seen || a.t[j].grid.data == a.t[i].grid.data;
+            seen = seen || T[j].grid.data == T[i].grid.data;
         if ( !seen )
             CA_HIP( ctx,
-                    launch_halo_adjoint_fill( *ctx, fp64, m, a.t[i].grid, w, s ),
+                    launch_halo_adjoint_fill( *ctx, fp64, m, T[i].grid, w, s ),
                     "halo adjoint fill" );
     }
     if ( n <= 0 )
         return CABANA_AMD_OK;
-    const bool mpm = num_terms == 3 && xbar && a.t[0].form == FORM_SV &&
-                     a.t[1].form == FORM_VV && a.t[2].form == FORM_TD;
+    const bool mpm = num_terms == 3 && xbar && T[0].form == FORM_SV &&
+                     T[1].form == FORM_VV && T[2].form == FORM_TD;
     if ( num_terms == 1 || mpm )
     {
         a.nterms = num_terms;
+        for ( int i = 0; i < num_terms; ++i )
+            a.t[i] = T[i];
         return gather_pass( ctx, order, fp64, mpm ? PLAN_MPM_P2G : PLAN_SINGLE,
                             a, n, bins, s );
     }
@@ -993,7 +1002,7 @@ extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
     {
         GatherArgs b = a;
         b.nterms = 1;
-        b.t[0] = a.t[i];
+        b.t[0] = T[i];
         CA_CHECK( gather_pass( ctx, order, fp64, PLAN_SINGLE, b, n, bins, s ) );
     }
     return CABANA_AMD_OK;
@@ -1011,9 +1020,9 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
     MeshDev m;
     CA_CHECK( make_mesh( ctx, mesh, m ) );
     CA_CHECK( check_order( ctx, order ) );
-    if ( !points || num_terms < 1 || num_terms > MAX_TERMS || !terms )
+    if ( !points || num_terms < 1 || num_terms > MAX_CALL_TERMS || !terms )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
-                     "need points and 1..3 terms" );
+                     "need points and 1..16 terms" );
     const int w = resolve_width( m, halo_width );
     CA_CHECK( check_width( ctx, m, w ) );
     const int64_t n = points->num_particle;
@@ -1025,6 +1034,7 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
 
     ScatterArgs a;
     std::memset( &a, 0, sizeof( a ) );
+    std::vector<ScatterTermDev> T( num_terms );
     a.mesh = m;
     a.skipped = ctx->skipped;
     CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, n, false,
@@ -1040,15 +1050,15 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
             return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                          "unknown G2P operator" );
         const int form = g2p_form( t.op );
-        a.t[i].form = form;
-        a.t[i].mult = t.multiplier;
+        T[i].form = form;
+        T[i].mult = t.multiplier;
         CA_CHECK( make_pfield( ctx, t.values_bar, form_pcomp( form ), rdt, n,
-                               false, "values_bar", a.t[i].pin ) );
+                               false, "values_bar", T[i].pin ) );
         CA_CHECK( make_gfield( ctx, m, t.grid_bar, form_gdof( form ), rdt,
-                               true, "grid_bar", a.t[i].grid ) );
+                               true, "grid_bar", T[i].grid ) );
         CA_CHECK( make_gfield( ctx, m, t.grid, form_gdof( form ), rdt, !xbar,
-                               "grid", a.t[i].gx ) );
-        if ( !a.t[i].grid.data && !xbar )
+                               "grid", T[i].gx ) );
+        if ( !T[i].grid.data && !xbar )
             return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                          "term has neither grid_bar nor points_bar" );
     }
@@ -1059,10 +1069,10 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
         {
             bool seen = false;
             for ( int j = 0; j < i; ++j )
-                seen = seen || a.t[j].gx.data == a.t[i].gx.data;
+                seen = seen || T[j].gx.data == T[i].gx.data;
             if ( !seen )
                 CA_HIP( ctx,
-                        launch_halo_gather( *ctx, fp64, m, a.t[i].gx, w, s ),
+                        launch_halo_gather( *ctx, fp64, m, T[i].gx, w, s ),
                         "halo gather" );
         }
     // Cotangent grids: clear ghosts, scatter, fold.
@@ -1070,9 +1080,9 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
     {
         bool seen = false;
         for ( int j = 0; j < i; ++j )
-            seen = seen || a.t[j].grid.data == a.t[i].grid.data;
-        if ( !seen && a.t[i].grid.data )
-            CA_HIP( ctx, launch_zero_ghosts( *ctx, fp64, m, a.t[i].grid, s ),
+            seen = seen || T[j].grid.data == T[i].grid.data;
+        if ( !seen && T[i].grid.data )
+            CA_HIP( ctx, launch_zero_ghosts( *ctx, fp64, m, T[i].grid, s ),
                     "zero ghosts" );
     }
     if ( n > 0 && xbar )
@@ -1086,19 +1096,23 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
         ga.points = a.points;
         ga.xbar = a.xbar;
         ga.has_xbar = 1;
+        std::vector<GatherTermDev> GT( num_terms );
         for ( int i = 0; i < num_terms; ++i )
         {
-            ga.t[i].form = a.t[i].form;
-            ga.t[i].mult = a.t[i].mult;
-            ga.t[i].px = a.t[i].pin;
-            ga.t[i].grid = a.t[i].gx;
+            std::memset( &GT[i], 0, sizeof( GT[i] ) );
+            GT[i].form = T[i].form;
+            GT[i].mult = T[i].mult;
+            GT[i].px = T[i].pin;
+            GT[i].grid = T[i].gx;
         }
-        const bool gpair = num_terms == 2 && a.t[0].form == FORM_VV &&
-                           a.t[1].form == FORM_TDT &&
-                           same_grid( a.t[0].gx, a.t[1].gx );
+        const bool gpair = num_terms == 2 && T[0].form == FORM_VV &&
+                           T[1].form == FORM_TDT &&
+                           same_grid( T[0].gx, T[1].gx );
         if ( num_terms == 1 || gpair )
         {
             ga.nterms = num_terms;
+            for ( int i = 0; i < num_terms; ++i )
+                ga.t[i] = GT[i];
             CA_CHECK( gather_pass( ctx, order, fp64,
                                    gpair ? PLAN_MPM_G2P : PLAN_SINGLE, ga, n,
                                    bins, s ) );
@@ -1109,7 +1123,7 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
             {
                 GatherArgs b = ga;
                 b.nterms = 1;
-                b.t[0] = ga.t[i];
+                b.t[0] = GT[i];
                 CA_CHECK( gather_pass( ctx, order, fp64, PLAN_SINGLE, b, n,
                                        bins, s ) );
             }
@@ -1119,13 +1133,15 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
     a.has_xbar = 0;
     if ( n > 0 )
     {
-        const bool pair = num_terms == 2 && a.t[0].form == FORM_VV &&
-                          a.t[1].form == FORM_TDT && a.t[0].grid.data &&
-                          same_grid( a.t[0].grid, a.t[1].grid );
+        const bool pair = num_terms == 2 && T[0].form == FORM_VV &&
+                          T[1].form == FORM_TDT && T[0].grid.data &&
+                          same_grid( T[0].grid, T[1].grid );
         if ( num_terms == 1 || pair )
         {
             a.nterms = num_terms;
-            if ( a.t[0].grid.data )
+            for ( int i = 0; i < num_terms; ++i )
+                a.t[i] = T[i];
+            if ( T[0].grid.data )
                 CA_CHECK( scatter_pass( ctx, order, fp64,
                                         pair ? PLAN_MPM_G2P : PLAN_SINGLE, a, n,
                                         bins, s ) );
@@ -1134,11 +1150,11 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
         {
             for ( int i = 0; i < num_terms; ++i )
             {
-                if ( !a.t[i].grid.data )
+                if ( !T[i].grid.data )
                     continue;
                 ScatterArgs b = a;
                 b.nterms = 1;
-                b.t[0] = a.t[i];
+                b.t[0] = T[i];
                 CA_CHECK( scatter_pass( ctx, order, fp64, PLAN_SINGLE, b, n,
                                         bins, s ) );
             }
@@ -1148,10 +1164,10 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
     {
         bool seen = false;
         for ( int j = 0; j < i; ++j )
-            seen = seen || a.t[j].grid.data == a.t[i].grid.data;
-        if ( !seen && a.t[i].grid.data )
+            seen = seen || T[j].grid.data == T[i].grid.data;
+        if ( !seen && T[i].grid.data )
             CA_HIP( ctx,
-                    launch_halo_scatter_sum( *ctx, fp64, m, a.t[i].grid, w, s ),
+                    launch_halo_scatter_sum( *ctx, fp64, m, T[i].grid, w, s ),
                     "halo scatter" );
     }
     return CABANA_AMD_OK;

@@ -106,7 +106,8 @@ inline int form_gdof( int form )
     return n[form];
 }
 
-constexpr int MAX_TERMS = 3;
+constexpr int MAX_TERMS = 3;       // terms fused into one kernel pass
+constexpr int MAX_CALL_TERMS = 16; // terms accepted by one API call
 
 // One term of a scatter pass (P2G forward, or G2P adjoint):
 //   grid += mult * dB/dG (P = pin)

@@ -176,8 +176,9 @@ cabana_amd_status cabana_amd_check_domain( cabana_amd_context* ctx,
 /* ----------------------------------------------------------------------- */
 
 /* p2g(functor, points, num_point, Spline<order>, halo, array)
- * (Cabana_Grid_Interpolation.hpp:1217-1313) for one or more functors in one
- * pass over the particles.  Each term ADDS its contributions into its grid
+ * (Cabana_Grid_Interpolation.hpp:1217-1313) for 1 to 16 functors (the MPM
+ * set ScalarValue + VectorValue + TensorDivergence runs in one pass over the
+ * particles, other sets in one pass per functor).  Each term ADDS its contributions into its grid
  * (owned and ghost nodes), then every distinct grid is folded once: the
  * ghost nodes within halo_width are summed into their periodic owners
  * (Halo::scatter with ScatterReduce::Sum).  Ghost nodes keep their partial
@@ -196,7 +197,7 @@ cabana_amd_status cabana_amd_p2g( cabana_amd_context* ctx,
                                   cabana_amd_stream stream );
 
 /* g2p(array, halo, points, num_point, Spline<order>, functor)
- * (Cabana_Grid_Interpolation.hpp:831-927) for one or more functors.  Every
+ * (Cabana_Grid_Interpolation.hpp:831-927) for 1 to 16 functors.  Every
  * distinct grid first gets its ghosts filled from their periodic owners
  * (Halo::gather), then each term ADDS multiplier * result into its particle
  * field. */

@@ -1,0 +1,465 @@
+// C++ drop-in layer (include/cabana_amd_grid.hpp) on the GPU, with minimal
+// stand-ins for the Cabana / Kokkos types it is duck-typed against (device
+// Views, an AoSoA slice, Grid arrays over a single-rank uniform mesh).
+//
+// Part 1 repeats Cabana's own interpolation unit test
+// (grid/unit_test/tstInterpolation3d.hpp:38-250): linear spline, one point
+// at every cell centre of a periodic mesh, constant fields; every P2G and
+// G2P operator must give -1.75 (values) or 0 (gradients, divergences).
+// Part 2 checks the call forms against each other on random data with the
+// quadratic spline: AoSoA slices vs Views, binned vs unbinned, fused vs
+// single functors, and the adjoints against the transpose identity.
+// Exit status 0 and a final "ALL PASSED" line mean success.
+#include "cabana_amd_grid.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <random>
+#include <vector>
+
+namespace cg = cabana_amd::grid;
+
+static int g_failures = 0;
+static int g_checks = 0;
+
+#define EXPECT_NEAR( a, b, tol )                                                            \
+    do                                                                                      \
+    {                                                                                       \
+        ++g_checks;                                                                         \
+        const double va_ = ( a ), vb_ = ( b );                                              \
+        if ( !( std::fabs( va_ - vb_ ) <= ( tol ) ) )                                        \
+        {                                                                                   \
+            if ( g_failures < 20 )                                                          \
+                std::printf( "FAIL %s:%d: %s = %.17g, expected %.17g\n", __FILE__, __LINE__, \
+                             #a, va_, vb_ );                                                \
+            ++g_failures;                                                                   \
+        }                                                                                   \
+    } while ( 0 )
+
+#define HIP_OK( call )                                                                      \
+    do                                                                                      \
+    {                                                                                       \
+        hipError_t e_ = ( call );                                                           \
+        if ( e_ != hipSuccess )                                                             \
+        {                                                                                   \
+            std::printf( "HIP error %s at %s:%d\n", hipGetErrorString( e_ ), __FILE__,       \
+                         __LINE__ );                                                        \
+            std::exit( 2 );                                                                 \
+        }                                                                                   \
+    } while ( 0 )
+
+// ---------------------------------------------------------------------------
+// Stand-ins
+// ---------------------------------------------------------------------------
+
+// Device view, LayoutRight, rank 1-4 (Kokkos::View interface subset).
+template <class T, int Rank>
+struct DeviceView
+{
+    static constexpr int rank = Rank;
+    std::shared_ptr<T> mem;
+    std::size_t ext[4] = { 1, 1, 1, 1 };
+    std::size_t str[4] = { 0, 0, 0, 0 };
+
+    DeviceView() = default;
+    explicit DeviceView( std::array<std::size_t, Rank> e )
+    {
+        std::size_t n = 1;
+        for ( int d = 0; d < Rank; ++d )
+        {
+            ext[d] = e[d];
+            n *= e[d];
+        }
+        std::size_t s = 1;
+        for ( int d = Rank - 1; d >= 0; --d )
+        {
+            str[d] = s;
+            s *= ext[d];
+        }
+        T* p = nullptr;
+        HIP_OK( hipMalloc( &p, n * sizeof( T ) + 1 ) );
+        HIP_OK( hipMemset( p, 0, n * sizeof( T ) + 1 ) );
+        mem = std::shared_ptr<T>( p, []( T* q ) { (void)hipFree( q ); } );
+    }
+    T* data() const { return mem.get(); }
+    std::size_t extent( int d ) const { return ext[d]; }
+    std::size_t stride( int d ) const { return str[d]; }
+    std::size_t size() const { return ext[0] * ext[1] * ext[2] * ext[3]; }
+
+    std::vector<T> to_host() const
+    {
+        std::vector<T> h( size() );
+        HIP_OK( hipDeviceSynchronize() );
+        HIP_OK( hipMemcpy( h.data(), data(), h.size() * sizeof( T ), hipMemcpyDeviceToHost ) );
+        return h;
+    }
+    void from_host( const std::vector<T>& h )
+    {
+        HIP_OK( hipMemcpy( data(), h.data(), h.size() * sizeof( T ), hipMemcpyHostToDevice ) );
+    }
+    void fill( T v ) { from_host( std::vector<T>( size(), v ) ); }
+};
+
+// One member of an AoSoA of struct { double pos[3]; double mass; double
+// vel[3]; } with vector length VL: the Cabana Slice interface subset.
+template <int VL, int NComp>
+struct AoSoASlice
+{
+    static constexpr int vector_length = VL;
+    std::shared_ptr<double> mem;
+    std::size_t offset;  // member offset within one SoA, in doubles
+    std::size_t n;
+    static constexpr std::size_t soa_doubles = 7 * VL;
+
+    double* data() const { return mem.get() + offset; }
+    std::size_t size() const { return n; }
+    std::size_t viewRank() const { return NComp == 1 ? 2 : 3; }
+    std::size_t extent( std::size_t d ) const
+    {
+        return d == 0 ? ( n + VL - 1 ) / VL : d == 1 ? VL : NComp;
+    }
+    std::size_t stride( std::size_t d ) const { return d == 0 ? soa_doubles : 1; }
+};
+
+struct GlobalMesh
+{
+    std::array<double, 3> lo, hi;
+    double cell;
+    double lowCorner( std::size_t d ) const { return lo[d]; }
+    double highCorner( std::size_t d ) const { return hi[d]; }
+    double cellSize( std::size_t ) const { return cell; }
+};
+struct GlobalGrid
+{
+    GlobalMesh mesh;
+    const GlobalMesh& globalMesh() const { return mesh; }
+    bool isPeriodic( int ) const { return true; }
+};
+struct LocalGrid
+{
+    GlobalGrid gg;
+    int halo;
+    const GlobalGrid& globalGrid() const { return gg; }
+    int haloCellWidth() const { return halo; }
+};
+struct ArrayLayout
+{
+    std::shared_ptr<LocalGrid> lg;
+    std::shared_ptr<LocalGrid> localGrid() const { return lg; }
+};
+template <class T>
+struct Array
+{
+    std::shared_ptr<ArrayLayout> lay;
+    DeviceView<T, 4> v;
+    std::shared_ptr<ArrayLayout> layout() const { return lay; }
+    const DeviceView<T, 4>& view() const { return v; }
+};
+
+template <class T>
+std::shared_ptr<Array<T>> make_array( std::shared_ptr<LocalGrid> lg, int dofs, int n[3] )
+{
+    auto a = std::make_shared<Array<T>>();
+    a->lay = std::make_shared<ArrayLayout>( ArrayLayout{ lg } );
+    const int h = lg->halo;
+    a->v = DeviceView<T, 4>( { std::size_t( n[0] + 2 * h + 1 ), std::size_t( n[1] + 2 * h + 1 ),
+                               std::size_t( n[2] + 2 * h + 1 ), std::size_t( dofs ) } );
+    return a;
+}
+
+// ---------------------------------------------------------------------------
+// Part 1: Cabana's interpolation test, through the C++ layer.
+// ---------------------------------------------------------------------------
+
+static void reference_interpolation_test()
+{
+    GlobalMesh gm{ { -1.2, 0.1, 1.1 }, { -0.3, 9.5, 2.3 }, 0.1 };
+    int n[3];
+    for ( int d = 0; d < 3; ++d )
+        n[d] = int( std::lrint( ( gm.hi[d] - gm.lo[d] ) / gm.cell ) );
+    const int h = 1;
+    auto lg = std::make_shared<LocalGrid>( LocalGrid{ GlobalGrid{ gm }, h } );
+
+    // A point at the centre of every owned cell.
+    const int num_point = n[0] * n[1] * n[2];
+    std::vector<double> ph( std::size_t( num_point ) * 3 );
+    for ( int i = 0; i < n[0]; ++i )
+        for ( int j = 0; j < n[1]; ++j )
+            for ( int k = 0; k < n[2]; ++k )
+            {
+                const int pid = i + n[0] * ( j + n[1] * k );
+                const int idx[3] = { i, j, k };
+                for ( int d = 0; d < 3; ++d )
+                    ph[3 * pid + d] = gm.lo[d] + ( idx[d] + 0.5 ) * gm.cell;
+            }
+    DeviceView<double, 2> points( { std::size_t( num_point ), 3 } );
+    points.from_host( ph );
+
+    auto scalar_grid = make_array<double>( lg, 1, n );
+    auto vector_grid = make_array<double>( lg, 3, n );
+    auto scalar_halo = cg::createHalo( cg::NodeHaloPattern<3>(), h, *scalar_grid );
+    auto vector_halo = cg::createHalo( cg::NodeHaloPattern<3>(), h, *vector_grid );
+
+    DeviceView<double, 1> scalar_pts( { std::size_t( num_point ) } );
+    DeviceView<double, 2> vector_pts( { std::size_t( num_point ), 3 } );
+    DeviceView<double, 3> tensor_pts( { std::size_t( num_point ), 3, 3 } );
+    scalar_pts.fill( 3.5 );
+    vector_pts.fill( 3.5 );
+    tensor_pts.fill( 3.5 );
+
+    // Owned nodes of a periodic mesh: [h, h + n).
+    auto for_owned = [&]( const DeviceView<double, 4>& v, auto&& fn ) {
+        auto host = v.to_host();
+        for ( int i = h; i < h + n[0]; ++i )
+            for ( int j = h; j < h + n[1]; ++j )
+                for ( int k = h; k < h + n[2]; ++k )
+                    for ( std::size_t d = 0; d < v.ext[3]; ++d )
+                        fn( host[i * v.str[0] + j * v.str[1] + k * v.str[2] + d] );
+    };
+
+    scalar_grid->v.fill( 0.0 );
+    cg::p2g( cg::createScalarValueP2G( scalar_pts, -0.5 ), points, num_point, cg::Spline<1>(),
+             *scalar_halo, *scalar_grid );
+    for_owned( scalar_grid->v, []( double x ) { EXPECT_NEAR( x, -1.75, 1e-13 ); } );
+
+    vector_grid->v.fill( 0.0 );
+    cg::p2g( cg::createVectorValueP2G( vector_pts, -0.5 ), points, num_point, cg::Spline<1>(),
+             *vector_halo, *vector_grid );
+    for_owned( vector_grid->v, []( double x ) { EXPECT_NEAR( x, -1.75, 1e-13 ); } );
+
+    vector_grid->v.fill( 0.0 );
+    cg::p2g( cg::createScalarGradientP2G( scalar_pts, -0.5 ), points, num_point,
+             cg::Spline<1>(), *vector_halo, *vector_grid );
+    for_owned( vector_grid->v, []( double x ) { EXPECT_NEAR( x + 1.0, 1.0, 1e-12 ); } );
+
+    scalar_grid->v.fill( 0.0 );
+    cg::p2g( cg::createVectorDivergenceP2G( vector_pts, -0.5 ), points, num_point,
+             cg::Spline<1>(), *scalar_halo, *scalar_grid );
+    for_owned( scalar_grid->v, []( double x ) { EXPECT_NEAR( x + 1.0, 1.0, 1e-12 ); } );
+
+    vector_grid->v.fill( 0.0 );
+    cg::p2g( cg::createTensorDivergenceP2G( tensor_pts, -0.5 ), points, num_point,
+             cg::Spline<1>(), *vector_halo, *vector_grid );
+    for_owned( vector_grid->v, []( double x ) { EXPECT_NEAR( x + 1.0, 1.0, 1e-12 ); } );
+
+    // G2P: owned nodes 3.5, ghosts garbage (the gather must overwrite them).
+    scalar_grid->v.fill( 99.0 );
+    vector_grid->v.fill( 99.0 );
+    {
+        auto fill_owned = [&]( DeviceView<double, 4>& v ) {
+            auto host = v.to_host();
+            for ( int i = h; i < h + n[0]; ++i )
+                for ( int j = h; j < h + n[1]; ++j )
+                    for ( int k = h; k < h + n[2]; ++k )
+                        for ( std::size_t d = 0; d < v.ext[3]; ++d )
+                            host[i * v.str[0] + j * v.str[1] + k * v.str[2] + d] = 3.5;
+            v.from_host( host );
+        };
+        fill_owned( scalar_grid->v );
+        fill_owned( vector_grid->v );
+    }
+    scalar_pts.fill( 0.0 );
+    cg::g2p( *scalar_grid, *scalar_halo, points, num_point, cg::Spline<1>(),
+             cg::createScalarValueG2P( scalar_pts, -0.5 ) );
+    for ( double x : scalar_pts.to_host() )
+        EXPECT_NEAR( x, -1.75, 1e-13 );
+
+    vector_pts.fill( 0.0 );
+    cg::g2p( *vector_grid, *vector_halo, points, num_point, cg::Spline<1>(),
+             cg::createVectorValueG2P( vector_pts, -0.5 ) );
+    for ( double x : vector_pts.to_host() )
+        EXPECT_NEAR( x, -1.75, 1e-13 );
+
+    vector_pts.fill( 0.0 );
+    cg::g2p( *scalar_grid, *scalar_halo, points, num_point, cg::Spline<1>(),
+             cg::createScalarGradientG2P( vector_pts, -0.5 ) );
+    for ( double x : vector_pts.to_host() )
+        EXPECT_NEAR( x + 1.0, 1.0, 1e-12 );
+
+    tensor_pts.fill( 0.0 );
+    cg::g2p( *vector_grid, *vector_halo, points, num_point, cg::Spline<1>(),
+             cg::createVectorGradientG2P( tensor_pts, -0.5 ) );
+    for ( double x : tensor_pts.to_host() )
+        EXPECT_NEAR( x + 1.0, 1.0, 1e-12 );
+
+    scalar_pts.fill( 0.0 );
+    cg::g2p( *vector_grid, *vector_halo, points, num_point, cg::Spline<1>(),
+             cg::createVectorDivergenceG2P( scalar_pts, -0.5 ) );
+    for ( double x : scalar_pts.to_host() )
+        EXPECT_NEAR( x + 1.0, 1.0, 1e-12 );
+    std::printf( "reference interpolation test: %d checks, %d failures\n", g_checks,
+                 g_failures );
+}
+
+// ---------------------------------------------------------------------------
+// Part 2: call forms against each other, quadratic spline, random data.
+// ---------------------------------------------------------------------------
+
+static double rel_diff( const std::vector<double>& a, const std::vector<double>& b )
+{
+    double num = 0, den = 0;
+    for ( std::size_t i = 0; i < a.size(); ++i )
+    {
+        num += ( a[i] - b[i] ) * ( a[i] - b[i] );
+        den += b[i] * b[i];
+    }
+    return std::sqrt( num / ( den > 0 ? den : 1 ) );
+}
+
+static void call_forms_test()
+{
+    const int cells = 16, h = 1;
+    int n[3] = { cells, cells, cells };
+    GlobalMesh gm{ { 0.0, 0.0, 0.0 }, { 16.0, 16.0, 16.0 }, 1.0 };
+    auto lg = std::make_shared<LocalGrid>( LocalGrid{ GlobalGrid{ gm }, h } );
+    const std::size_t np = 20000;
+    constexpr int VL = 16;
+    std::mt19937_64 rng( 1234 );
+    std::uniform_real_distribution<double> U( 0.0, 16.0 );
+    std::normal_distribution<double> Nd( 0.0, 1.0 );
+
+    // AoSoA { pos[3], mass, vel[3] } and the same data as Views.
+    const std::size_t nsoa = ( np + VL - 1 ) / VL;
+    std::vector<double> aosoa( nsoa * 7 * VL, 0.0 );
+    std::vector<double> pos( np * 3 ), mass( np ), vel( np * 3 );
+    for ( std::size_t p = 0; p < np; ++p )
+    {
+        const std::size_t s = p / VL, a = p % VL;
+        for ( int d = 0; d < 3; ++d )
+        {
+            pos[3 * p + d] = U( rng );
+            vel[3 * p + d] = Nd( rng );
+            aosoa[s * 7 * VL + d * VL + a] = pos[3 * p + d];
+            aosoa[s * 7 * VL + ( 4 + d ) * VL + a] = vel[3 * p + d];
+        }
+        mass[p] = 0.5 + U( rng ) / 16.0;
+        aosoa[s * 7 * VL + 3 * VL + a] = mass[p];
+    }
+    double* dev = nullptr;
+    HIP_OK( hipMalloc( &dev, aosoa.size() * sizeof( double ) ) );
+    HIP_OK( hipMemcpy( dev, aosoa.data(), aosoa.size() * sizeof( double ),
+                       hipMemcpyHostToDevice ) );
+    std::shared_ptr<double> mem( dev, []( double* q ) { (void)hipFree( q ); } );
+    AoSoASlice<VL, 3> s_pos{ mem, 0, np };
+    AoSoASlice<VL, 1> s_mass{ mem, 3 * VL, np };
+    AoSoASlice<VL, 3> s_vel{ mem, 4 * VL, np };
+
+    DeviceView<double, 2> v_pos( { np, 3 } ), v_vel( { np, 3 } );
+    DeviceView<double, 1> v_mass( { np } );
+    v_pos.from_host( pos );
+    v_vel.from_host( vel );
+    v_mass.from_host( mass );
+
+    auto gm_a = make_array<double>( lg, 1, n ), gm_b = make_array<double>( lg, 1, n );
+    auto gq_a = make_array<double>( lg, 3, n ), gq_b = make_array<double>( lg, 3, n );
+    auto halo = cg::createHalo( cg::NodeHaloPattern<3>(), h, *gm_a );
+
+    // Slices, one functor per call.
+    cg::p2g( cg::createScalarValueP2G( s_mass, 1.0 ), s_pos, np, cg::Spline<2>(), *halo, *gm_a );
+    cg::p2g( cg::createVectorValueP2G( s_vel, 0.5 ), s_pos, np, cg::Spline<2>(), *halo, *gq_a );
+    // Views, fused.
+    cg::p2g( cg::detail::DefaultExec{},
+             std::vector<cg::P2GBinding>{
+                 cg::bind( cg::createScalarValueP2G( v_mass, 1.0 ), *gm_b ),
+                 cg::bind( cg::createVectorValueP2G( v_vel, 0.5 ), *gq_b ) },
+             v_pos, np, cg::Spline<2>(), *halo, *gm_b );
+    const double e1 = rel_diff( gm_a->v.to_host(), gm_b->v.to_host() );
+    const double e2 = rel_diff( gq_a->v.to_host(), gq_b->v.to_host() );
+    EXPECT_NEAR( e1, 0.0, 1e-13 );
+    EXPECT_NEAR( e2, 0.0, 1e-13 );
+
+    // Binned: LinkedCellList + permute of the View copies, then p2g with the
+    // binning; owned nodes must match the unbinned result.
+    auto lcl = cg::createLinkedCellList( v_pos, { 1.0, 1.0, 1.0 }, { 0.0, 0.0, 0.0 },
+                                         { 16.0, 16.0, 16.0 } );
+    cg::permute( *lcl, v_pos );
+    cg::permute( *lcl, v_mass );
+    cg::permute( *lcl, v_vel );
+    auto gm_c = make_array<double>( lg, 1, n );
+    cg::p2g( cg::createScalarValueP2G( v_mass, 1.0 ), v_pos, np, cg::Spline<2>(), *halo, *gm_c,
+             lcl.get() );
+    {
+        auto a = gm_a->v.to_host(), c = gm_c->v.to_host();
+        std::vector<double> ao, co;
+        const auto& v = gm_a->v;
+        for ( int i = h; i < h + cells; ++i )
+            for ( int j = h; j < h + cells; ++j )
+                for ( int k = h; k < h + cells; ++k )
+                {
+                    ao.push_back( a[i * v.str[0] + j * v.str[1] + k * v.str[2]] );
+                    co.push_back( c[i * v.str[0] + j * v.str[1] + k * v.str[2]] );
+                }
+        EXPECT_NEAR( rel_diff( co, ao ), 0.0, 1e-13 );
+    }
+    // LinkedCellList accessors: counts sum to np, offsets are their scan.
+    {
+        long long total = 0;
+        bool scan_ok = true;
+        for ( int i = 0; i < lcl->numBin( 0 ); ++i )
+            for ( int j = 0; j < lcl->numBin( 1 ); ++j )
+                for ( int k = 0; k < lcl->numBin( 2 ); ++k )
+                {
+                    scan_ok = scan_ok && lcl->binOffset( i, j, k ) == total;
+                    total += lcl->binSize( i, j, k );
+                }
+        EXPECT_NEAR( double( total ), double( np ), 0.0 );
+        EXPECT_NEAR( scan_ok ? 1.0 : 0.0, 1.0, 0.0 );
+        EXPECT_NEAR( double( lcl->rangeEnd() - lcl->rangeBegin() ), double( np ), 0.0 );
+    }
+
+    // Transpose identity through the adjoint entry: <P2G(m), g>_own equals
+    // <m, p2g_adjoint(g)>.
+    auto g = make_array<double>( lg, 1, n );
+    std::vector<double> gh( g->v.size() );
+    for ( auto& x : gh )
+        x = Nd( rng );
+    g->v.from_host( gh );
+    DeviceView<double, 1> mbar( { np } );
+    cg::p2g_adjoint( cg::createScalarValueP2G( v_mass, 1.0 ), v_pos, np, cg::Spline<2>(), *halo,
+                     *g, mbar, static_cast<const DeviceView<double, 2>*>( nullptr ) );
+    double lhs = 0, rhs = 0;
+    {
+        auto c = gm_c->v.to_host();
+        const auto& v = gm_c->v;
+        for ( int i = h; i < h + cells; ++i )
+            for ( int j = h; j < h + cells; ++j )
+                for ( int k = h; k < h + cells; ++k )
+                {
+                    const std::size_t o = i * v.str[0] + j * v.str[1] + k * v.str[2];
+                    lhs += c[o] * gh[o];
+                }
+        auto mb = mbar.to_host();
+        auto ms = v_mass.to_host();
+        for ( std::size_t p = 0; p < np; ++p )
+            rhs += ms[p] * mb[p];
+    }
+    EXPECT_NEAR( ( lhs - rhs ) / std::fabs( rhs ), 0.0, 1e-12 );
+
+    // Errors become exceptions: a VectorValueP2G on a scalar field.
+    bool threw = false;
+    try
+    {
+        (void)cg::createVectorValueP2G( v_mass, 1.0 );
+    }
+    catch ( const cg::Error& )
+    {
+        threw = true;
+    }
+    EXPECT_NEAR( threw ? 1.0 : 0.0, 1.0, 0.0 );
+    std::printf( "call forms: slice/view %.2e %.2e, binned/unbinned ok, transpose %.2e\n", e1,
+                 e2, ( lhs - rhs ) / std::fabs( rhs ) );
+}
+
+int main()
+{
+    reference_interpolation_test();
+    call_forms_test();
+    HIP_OK( hipDeviceSynchronize() );
+    std::printf( "%d checks, %d failures\n", g_checks, g_failures );
+    if ( g_failures )
+        return 1;
+    std::printf( "ALL PASSED\n" );
+    return 0;
+}
