@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--mode", choices=["both", "fp32", "fp64"], default="both")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the differentiation and checkpointed-loop figures")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one elementwise copy (y = 2*x, FP32) of this many MiB: "
                          "a kernel of known traffic against which rocprofv3 FETCH_SIZE and "
@@ -150,6 +152,71 @@ class Workload:
         positions and the 13 particle attributes."""
         self.lcl.build(self.pos, 0, self.n)
         self.ca.permute(self.lcl, self.pos, self.m, self.q, self.sig)
+
+
+def differentiation_figure(torch, ca, steps=5, warmup=2, cells=64, ppc=8, seed=404):
+    """64^3 periodic cells x 8 ppc, quadratic: L = sum_owned P2G(v)^2 +
+    sum_p |G2P(g)(x_p)|^2 through the autograd wrappers, FP32-mixed; one
+    step = forward + backward (value and position gradients)."""
+    from cabana_amd import autograd as AG
+    dev = torch.device("cuda", 0)
+    n = cells ** 3 * ppc
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    gm = ca.createUniformGlobalMesh([0.0] * 3, [float(cells)] * 3, 1.0)
+    lg = ca.createLocalGrid(ca.createGlobalGrid(gm, [True] * 3), 1)
+    x = torch.rand(n, 3, dtype=torch.float64, device=dev, generator=g) * cells
+    lcl = ca.createLinkedCellList(x, [1.0] * 3, [0.0] * 3, [float(cells)] * 3)
+    x = x[lcl.permutes].contiguous()
+    v = torch.randn(n, dtype=torch.float32, device=dev, generator=g)
+    gf = torch.randn(cells, cells, cells, 3, dtype=torch.float32, device=dev, generator=g)
+
+    def step():
+        xr = x.detach().requires_grad_()
+        vr = v.detach().requires_grad_()
+        grid = AG.p2g(xr, vr, lg, "ScalarValue", order=2, bins=lcl)
+        u = AG.g2p(xr, gf, lg, "VectorValue", order=2, bins=lcl)
+        loss = (grid * grid).sum() + (u * u).sum()
+        loss.backward()
+
+    ms = time_steps(torch, step, steps, warmup)
+    return {"workload": "64^3 cells x 8 ppc = %d particles, Spline<2>, "
+                        "L = sum P2G(v)^2 + sum |G2P(g)|^2, autograd fwd+bwd" % n,
+            "dtype": "f32-mixed", "ms_per_step": ms, "particles_per_s": n / (ms * 1e-3)}
+
+
+def loop_figure(torch, ca, iters=1, warmup=1, cells=256, num_particle=1 << 26, steps=10,
+                every=2, seed=505):
+    """Checkpointed explicit MPM-like loop: 256^3 periodic cells, 2^26
+    particles in 32 Gaussian clusters (sigma 4-16 cells) over a 10% uniform
+    background, FP32-mixed, quadratic; 10 steps (re-bin, P2G mass+momentum,
+    grid velocity update, G2P velocity+gradient, advection), then backward
+    through all steps with a checkpoint every 2 steps."""
+    from cabana_amd import mpm
+    box = mpm.PeriodicBox(cells)
+    x0 = mpm.clustered_positions(num_particle, cells, seed=seed)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed + 1)
+    v0 = 0.5 * torch.randn(num_particle, 3, dtype=torch.float32, device="cuda", generator=g)
+    m = torch.rand(num_particle, dtype=torch.float32, device="cuda", generator=g) + 0.5
+
+    def run():
+        x = x0.detach().requires_grad_()
+        v = v0.detach().requires_grad_()
+        xs, vs, _, _ = mpm.run_loop(box, x, v, m, num_steps=steps, checkpoint_every=every)
+        loss = (vs.double() ** 2).sum() + 1e-3 * xs.sum()
+        loss.backward()
+
+    ms = time_steps(torch, run, iters, warmup)
+    counts = torch.bincount(
+        ((x0[:, 0].long() * cells + x0[:, 1].long()) * cells + x0[:, 2].long()),
+        minlength=cells ** 3)
+    out = {"workload": "256^3 cells, %d particles (32 Gaussian clusters + 10%% uniform, "
+                       "max %d per cell), Spline<2>, %d steps fwd + checkpointed bwd "
+                       "(every %d)" % (num_particle, int(counts.max()), steps, every),
+           "dtype": "f32-mixed", "ms_fwd_bwd": ms,
+           "particle_steps_per_s": num_particle * steps / (ms * 1e-3)}
+    return out
 
 
 def time_steps(torch, fn, steps, warmup):
@@ -316,6 +383,14 @@ def main():
                      for k, v in kernels.items() if algorithmic_bytes(k, 1, s, 1)) / head["prof_steps"]
     roofline["step_algorithmic_bytes"] = step_bytes
     roofline["step_frac"] = step_bytes / (head["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+    extra = {}
+    if not args.no_extra and "fp32" in res and args.cells == 128:
+        extra["differentiation"] = differentiation_figure(torch, ca)
+        torch.cuda.empty_cache()
+        extra["checkpointed_loop"] = loop_figure(torch, ca)
+        torch.cuda.empty_cache()
+        for k, v in extra.items():
+            print("[bench] %s: %s" % (k, json.dumps(v)), file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cells, args.ppc, args.order, args.cpu_seconds)
@@ -348,6 +423,7 @@ def main():
     out["value_with_sort"] = head["n"] / ((head["ms"] + head["sort_ms"]) * 1e-3)
     out["kernels"] = {k: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"]}
                       for k, v in kernels.items()}
+    out.update(extra)
     print(json.dumps(out))
 
 

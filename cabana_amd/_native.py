@@ -164,11 +164,13 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    # CABANA_AMD_LIBRARY selects another build of the same library.
+    path = os.environ.get("CABANA_AMD_LIBRARY", LIB_PATH)
+    if not os.path.exists(path):
         raise RuntimeError(
             "cabana_amd: %s is missing; build it with `python -m cabana_amd.build` "
-            "(there is no fallback path)" % LIB_PATH)
-    lib = ctypes.CDLL(LIB_PATH)
+            "(there is no fallback path)" % path)
+    lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res

@@ -38,8 +38,8 @@ struct cabana_amd_context : public Launcher
     unsigned long long* bad = nullptr;     // device counter (binning)
 
     // Reusable scratch.
-    void* scratch[4] = { nullptr, nullptr, nullptr, nullptr };
-    size_t scratch_bytes[4] = { 0, 0, 0, 0 };
+    void* scratch[5] = { nullptr, nullptr, nullptr, nullptr, nullptr };
+    size_t scratch_bytes[5] = { 0, 0, 0, 0, 0 };
 
     // Kernel timing.
     bool profiling = false;
@@ -134,7 +134,8 @@ enum Slot
     SLOT_CELL_OF = 0,
     SLOT_CURSOR = 1,
     SLOT_SCAN = 2,
-    SLOT_PERMUTE = 3
+    SLOT_PERMUTE = 3,
+    SLOT_TILES = 4
 };
 
 cabana_amd_status fail( cabana_amd_context* ctx, cabana_amd_status st,
@@ -262,14 +263,28 @@ cabana_amd_status make_pfield( cabana_amd_context* ctx,
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                      std::string( what ) +
                          ": soa_stride smaller than vector_length * num_comp" );
+    // Device addressing is 32-bit: particle indices below 2^31 and every
+    // element index of the field below 2^32.
+    {
+        const int64_t np = f.num_particle;
+        const int64_t vl = f.vector_length;
+        const int64_t last = np > 0 ? ( ( np - 1 ) / vl ) * f.soa_stride +
+                                          ( np - 1 ) % vl + vl * ( f.num_comp - 1 )
+                                    : vl * ( f.num_comp - 1 );
+        if ( np >= ( int64_t( 1 ) << 31 ) || f.soa_stride < 0 || last >= ( int64_t( 1 ) << 32 ) ||
+             vl >= ( int64_t( 1 ) << 32 ) )
+            return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED,
+                         std::string( what ) +
+                             ": more than 2^31 particles or 2^32 elements in one field" );
+    }
     out.data = static_cast<char*>( f.data );
-    out.vl = f.vector_length;
-    out.stride = f.soa_stride;
+    out.vl = (uint32_t)f.vector_length;
+    out.stride = (uint32_t)f.soa_stride;
     out.ncomp = f.num_comp;
     if ( f.vector_length >= f.num_particle )
     {
-        out.shift = 62;
-        out.mask = ( int64_t( 1 ) << 62 ) - 1;
+        out.shift = 31;
+        out.mask = 0x7fffffffu;
     }
     else if ( ( f.vector_length & ( f.vector_length - 1 ) ) == 0 )
     {
@@ -277,12 +292,13 @@ cabana_amd_status make_pfield( cabana_amd_context* ctx,
         while ( ( int64_t( 1 ) << sh ) < f.vector_length )
             ++sh;
         out.shift = sh;
-        out.mask = f.vector_length - 1;
+        out.mask = (uint32_t)( f.vector_length - 1 );
     }
     else
     {
-        out.shift = -1;
-        out.mask = 0;
+        return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED,
+                     std::string( what ) +
+                         ": vector_length must be a power of two (or cover every particle)" );
     }
     return CABANA_AMD_OK;
 }
@@ -444,18 +460,33 @@ enum TileKind
     TILE_SCATTER = 1
 };
 
+constexpr int TILE_BLOCK = 256;
+constexpr int SCATTER_SUB_LEN = 8;
+
+struct TileShape
+{
+    int T[3];
+    int cap;
+    int max_tasks;
+};
+
+int base_cells( const int T[3], int order )
+{
+    const int spread = ( order == 2 ) ? 1 : 0;
+    return ( T[0] + spread ) * ( T[1] + spread ) * ( T[2] + spread );
+}
+
 // LDS bytes of the tiled kernels; mirrors their in-kernel layout.
-size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng,
-                       int nslot, int T, int order, int cap )
+size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng, int nslot,
+                       int rec_len, int order, const TileShape& sh )
 {
     const int extra = ( order == 1 ) ? 1 : 3;
-    const size_t w = T + extra;
-    const size_t wn = w * w * w;
-    const size_t runs = (size_t)T * T;
+    const size_t wn = (size_t)( sh.T[0] + extra ) * ( sh.T[1] + extra ) * ( sh.T[2] + extra );
+    const size_t runs = (size_t)sh.T[0] * sh.T[1];
+    const size_t real = fp64 ? 8 : 4;
     size_t b = 0;
     if ( kind == TILE_GATHER )
     {
-        const size_t real = fp64 ? 8 : 4;
         for ( int i = 0; i < nslot; ++i )
             b = a16( b + real * wn * ( slot_ng[i] == 1 ? 1 : 4 ) );
         b = a16( b + 8 * runs );
@@ -465,22 +496,99 @@ size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng,
     int dt = 0;
     for ( int i = 0; i < nslot; ++i )
         dt += slot_ng[i];
-    const size_t nbase = (size_t)( T + ( order == 2 ? 1 : 0 ) );
+    const size_t nbase = (size_t)base_cells( sh.T, order );
+    const size_t rs = (size_t)record_stride( rec_len, (int)real );
     b = a16( 8 * (size_t)dt * wn );
-    b = a16( b + 4 * ( nbase * nbase * nbase + 1 ) );
-    b = a16( b + 4 * (size_t)cap );
-    b = a16( b + 4 * (size_t)cap );
+    b = a16( b + 4 * ( nbase + 1 ) );
+    b = a16( b + 4 * ( nbase + 1 ) );
+    b = a16( b + 4 * (size_t)sh.max_tasks );
+    b = a16( b + real * rs * (size_t)sh.cap );
     b = a16( b + 8 * runs );
     b = a16( b + 4 * ( runs + 1 ) );
-    b += 16;
+    b += 4 * ( TILE_BLOCK / 64 ) + 16;
     return b;
 }
 
+// Brick shape and part capacity.  Gathers use the largest cubic brick whose
+// windows fit three workgroups per CU, with parts of ~1.25x the mean brick
+// population.  Scatters sort a whole part in LDS: the brick is the largest
+// near-cubic one whose expected population (x1.3) fits a part that, with the
+// FP64 window, keeps two workgroups per CU; denser bricks are cut into
+// several parts.
+bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int rec_len,
+                   int order, double mean_ppc, TileShape& out )
+{
+    if ( kind == TILE_GATHER )
+    {
+        static const int cand[] = { 8, 6, 4, 3, 2 };
+        for ( int c : cand )
+        {
+            TileShape sh = { { c, c, c }, 0, 0 };
+            if ( tile_lds_bytes( kind, fp64, slot_ng, nslot, rec_len, order, sh ) <=
+                 52 * 1024 )
+            {
+                const double mean = mean_ppc * c * c * c;
+                double cap = 1.25 * mean;
+                cap = cap < 4096 ? 4096 : ( cap > ( 1 << 20 ) ? ( 1 << 20 ) : cap );
+                sh.cap = ( (int)cap + 255 ) / 256 * 256;
+                out = sh;
+                return true;
+            }
+        }
+        return false;
+    }
+    const size_t budget = 80 * 1024;
+    auto with_cap = [&]( TileShape sh, int cap ) {
+        sh.cap = cap;
+        sh.max_tasks = base_cells( sh.T, order ) + cap / SCATTER_SUB_LEN + 1;
+        return sh;
+    };
+    bool found = false;
+    TileShape best = {};
+    int best_vol = 0;
+    for ( int a = 2; a <= 8; ++a )
+        for ( int b = a; b <= a + 1 && b <= 8; ++b )
+            for ( int c = b; c <= b + 1 && c <= 8; ++c )
+            {
+                const int vol = a * b * c;
+                if ( vol <= best_vol )
+                    continue;
+                int cap = (int)( 1.3 * mean_ppc * vol ) + 32;
+                cap = ( cap + 63 ) / 64 * 64;
+                cap = cap < 256 ? 256 : cap;
+                if ( cap > SCATTER_CAP_MAX )
+                    continue;
+                TileShape sh = with_cap( { { a, b, c }, 0, 0 }, cap );
+                if ( tile_lds_bytes( kind, fp64, slot_ng, nslot, rec_len, order, sh ) <=
+                     budget )
+                {
+                    best = sh;
+                    best_vol = vol;
+                    found = true;
+                }
+            }
+    if ( !found )
+    {
+        // Dense: the smallest brick with the largest part that fits.
+        for ( int cap = SCATTER_CAP_MAX; cap >= 64; cap -= 64 )
+        {
+            TileShape sh = with_cap( { { 2, 2, 2 }, 0, 0 }, cap );
+            if ( tile_lds_bytes( kind, fp64, slot_ng, nslot, rec_len, order, sh ) <= budget )
+            {
+                best = sh;
+                found = true;
+                break;
+            }
+        }
+    }
+    out = best;
+    return found;
+}
+
 // Tile parameters for cell-sorted particles; false when the binning does not
-// describe the mesh's own cells.  The tile edge is the largest that keeps the
-// LDS footprint within the budget of two (gather: three) workgroups per CU.
+// describe the mesh's own cells.
 bool make_tiles( TileKind kind, const MeshDev& m, int order, bool fp64,
-                 const int* slot_ng, int nslot,
+                 const int* slot_ng, int nslot, int rec_len,
                  const cabana_amd_binning* bins, const GFieldDev& g0,
                  TileArgs& t, size_t& lds )
 {
@@ -489,44 +597,27 @@ bool make_tiles( TileKind kind, const MeshDev& m, int order, bool fp64,
     for ( int d = 0; d < 3; ++d )
         if ( bins->num_bin[d] != m.num_cell[d] )
             return false;
+    const double ncell = (double)m.num_cell[0] * m.num_cell[1] * m.num_cell[2];
+    const double mean_ppc = ncell > 0 ? (double)( bins->end - bins->begin ) / ncell : 0.0;
+    TileShape sh;
+    if ( !choose_shape( kind, fp64, slot_ng, nslot, rec_len, order, mean_ppc, sh ) )
+        return false;
     const int extra = ( order == 1 ) ? 1 : 3;
-    const size_t budget = ( kind == TILE_GATHER ) ? 52 * 1024 : 80 * 1024;
-    static const int cand[] = { 8, 6, 4, 3, 2 };
-    int T = 2, cap = 1024;
-    for ( int c : cand )
-    {
-        int cp = 1024;
-        if ( kind == TILE_SCATTER )
-        {
-            // Room for the whole brick at up to 20 particles per cell;
-            // denser bricks run in several batches.
-            cp = 20 * c * c * c;
-            cp = cp < 1024 ? 1024 : ( cp > 8192 ? 8192 : cp );
-        }
-        if ( tile_lds_bytes( kind, fp64, slot_ng, nslot, c, order, cp ) <= budget )
-        {
-            T = c;
-            cap = cp;
-            break;
-        }
-    }
     std::memset( &t, 0, sizeof( t ) );
     t.cell_offsets = bins->offsets;
     t.cell_counts = bins->counts;
-    int ntiles = 1;
     for ( int d = 0; d < 3; ++d )
     {
-        t.tile[d] = T;
-        t.ntile[d] = ( m.num_cell[d] + T - 1 ) / T;
-        t.win[d] = T + extra;
-        ntiles *= t.ntile[d];
+        t.tile[d] = sh.T[d];
+        t.ntile[d] = ( m.num_cell[d] + sh.T[d] - 1 ) / sh.T[d];
+        t.win[d] = sh.T[d] + extra;
     }
-    t.tiles_per_xcd = ( ntiles + 7 ) / 8;
-    t.max_runs = T * T;
+    t.max_runs = sh.T[0] * sh.T[1];
     t.max_win_nodes = t.win[0] * t.win[1] * t.win[2];
-    const int nbase = T + ( order == 2 ? 1 : 0 );
-    t.max_base_cells = nbase * nbase * nbase;
-    t.cap = cap;
+    t.max_base_cells = base_cells( sh.T, order );
+    t.cap = sh.cap;
+    t.sub_len = SCATTER_SUB_LEN;
+    t.max_tasks = sh.max_tasks;
     t.bin_begin = bins->begin;
     // Window walk order: largest grid stride first.
     int perm[3] = { 0, 1, 2 };
@@ -540,8 +631,47 @@ bool make_tiles( TileKind kind, const MeshDev& m, int order, bool fp64,
             }
     for ( int d = 0; d < 3; ++d )
         t.flush_perm[d] = perm[d];
-    lds = tile_lds_bytes( kind, fp64, slot_ng, nslot, T, order, cap );
+    lds = tile_lds_bytes( kind, fp64, slot_ng, nslot, rec_len, order, sh );
     return true;
+}
+
+// Builds the part table of a tiling in the context scratch and sets the
+// launch grid.
+cabana_amd_status make_parts( cabana_amd_context* ctx, const MeshDev& m,
+                              const cabana_amd_binning* bins, TileArgs& t,
+                              hipStream_t s )
+{
+    const int64_t nbricks = (int64_t)t.ntile[0] * t.ntile[1] * t.ntile[2];
+    const int64_t nbin = bins->end - bins->begin;
+    const int64_t max_parts = nbricks + ( nbin + t.cap - 1 ) / t.cap;
+    const int64_t nscan = ( nbricks + 1023 ) / 1024 + 1;
+    size_t off = 0;
+    auto carve = [&]( size_t bytes ) {
+        const size_t o = off;
+        off = ( off + bytes + 255 ) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_np = carve( sizeof( int32_t ) * nbricks );
+    const size_t o_st = carve( sizeof( int64_t ) * nbricks );
+    const size_t o_bs = carve( sizeof( int64_t ) * nscan );
+    const size_t o_pt = carve( sizeof( int2 ) * max_parts );
+    const size_t o_n = carve( sizeof( int32_t ) );
+    CA_HIP( ctx, ctx->ensure( SLOT_TILES, off ), "scratch" );
+    char* base = static_cast<char*>( ctx->scratch[SLOT_TILES] );
+    int2* parts = reinterpret_cast<int2*>( base + o_pt );
+    int32_t* num_parts = reinterpret_cast<int32_t*>( base + o_n );
+    CA_HIP( ctx,
+            launch_brick_parts( *ctx, t, m.num_cell,
+                                reinterpret_cast<int32_t*>( base + o_np ),
+                                reinterpret_cast<int64_t*>( base + o_st ),
+                                reinterpret_cast<int64_t*>( base + o_bs ), nscan, parts,
+                                num_parts, s ),
+            "brick parts" );
+    t.parts = parts;
+    t.num_parts = num_parts;
+    int64_t grid = ( max_parts + 7 ) / 8 * 8;
+    t.grid = (int32_t)( grid < 4096 ? grid : 4096 );
+    return CABANA_AMD_OK;
 }
 
 bool bins_usable( const cabana_amd_binning* bins, int64_t n )
@@ -574,11 +704,14 @@ cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
     for ( int i = 0; i < a.nterms; ++i )
         forms[i] = a.t[i].form;
     const int nslot = slot_dofs( plan, forms, a.nterms, ng );
+    int rec_len = 3;
+    for ( int i = 0; i < a.nterms; ++i )
+        rec_len += form_pcomp( forms[i] );
     TileArgs t;
     size_t lds = 0;
     const bool tiled =
         !a.has_xbar && bins_usable( bins, n ) &&
-        make_tiles( TILE_SCATTER, a.mesh, order, fp64, ng, nslot, bins,
+        make_tiles( TILE_SCATTER, a.mesh, order, fp64, ng, nslot, rec_len, bins,
                     a.t[0].grid, t, lds );
     if ( !tiled )
     {
@@ -596,8 +729,11 @@ cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
                 "scatter (atomic)" );
     }
     if ( bins->end > bins->begin )
+    {
+        CA_CHECK( make_parts( ctx, a.mesh, bins, t, s ) );
         CA_HIP( ctx, run_scatter_cells( ctx, order, fp64, plan, a, t, lds, s ),
                 "scatter (cells)" );
+    }
     if ( bins->end < n )
     {
         a.begin = bins->end;
@@ -622,7 +758,7 @@ cabana_amd_status gather_pass( cabana_amd_context* ctx, int order, bool fp64,
     size_t lds = 0;
     const bool tiled =
         bins_usable( bins, n ) &&
-        make_tiles( TILE_GATHER, a.mesh, order, fp64, ng, nslot, bins,
+        make_tiles( TILE_GATHER, a.mesh, order, fp64, ng, nslot, 0, bins,
                     a.t[0].grid, t, lds );
     if ( !tiled )
     {
@@ -638,8 +774,11 @@ cabana_amd_status gather_pass( cabana_amd_context* ctx, int order, bool fp64,
         CA_HIP( ctx, run_gather( ctx, order, fp64, plan, a, s ), "gather" );
     }
     if ( bins->end > bins->begin )
+    {
+        CA_CHECK( make_parts( ctx, a.mesh, bins, t, s ) );
         CA_HIP( ctx, run_gather_tiled( ctx, order, fp64, plan, a, t, lds, s ),
                 "gather (tiled)" );
+    }
     if ( bins->end < n )
     {
         a.begin = bins->end;
@@ -647,6 +786,23 @@ cabana_amd_status gather_pass( cabana_amd_context* ctx, int order, bool fp64,
         CA_HIP( ctx, run_gather( ctx, order, fp64, plan, a, s ), "gather" );
     }
     return CABANA_AMD_OK;
+}
+
+// True when the first n particles of two fields share memory.  Fused gather
+// passes read every output of a particle before writing any of them, so
+// terms whose outputs overlap run as separate passes.
+bool pfields_overlap( const PFieldDev& x, const PFieldDev& y, int64_t n, int elem )
+{
+    if ( !x.data || !y.data || n <= 0 )
+        return false;
+    auto span = [&]( const PFieldDev& f ) {
+        const int64_t last = ( ( n - 1 ) / (int64_t)f.vl ) * (int64_t)f.stride +
+                             ( n - 1 ) % (int64_t)f.vl + (int64_t)f.vl * ( f.ncomp - 1 );
+        return ( last + 1 ) * (int64_t)elem;
+    };
+    const char* x0 = x.data;
+    const char* y0 = y.data;
+    return x0 < y0 + span( y ) && y0 < x0 + span( x );
 }
 
 cabana_amd_status validate_bins( cabana_amd_context* ctx,
@@ -713,7 +869,7 @@ extern "C" cabana_amd_status cabana_amd_context_destroy( cabana_amd_context* ctx
     ctx->drain();
     for ( auto e : ctx->pool )
         hipEventDestroy( e );
-    for ( int i = 0; i < 4; ++i )
+    for ( int i = 0; i < 5; ++i )
         if ( ctx->scratch[i] )
             hipFree( ctx->scratch[i] );
     if ( ctx->skipped )
@@ -901,7 +1057,8 @@ extern "C" cabana_amd_status cabana_amd_g2p(
         return CABANA_AMD_OK;
     const bool pair = num_terms == 2 && T[0].form == FORM_VV &&
                       T[1].form == FORM_TDT &&
-                      same_grid( T[0].grid, T[1].grid );
+                      same_grid( T[0].grid, T[1].grid ) &&
+                      !pfields_overlap( T[0].pout, T[1].pout, n, fp64 ? 8 : 4 );
     if ( num_terms == 1 || pair )
     {
         a.nterms = num_terms;
@@ -989,7 +1146,11 @@ extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
     if ( n <= 0 )
         return CABANA_AMD_OK;
     const bool mpm = num_terms == 3 && xbar && T[0].form == FORM_SV &&
-                     T[1].form == FORM_VV && T[2].form == FORM_TD;
+                     T[1].form == FORM_VV && T[2].form == FORM_TD && T[0].pout.data &&
+                     T[1].pout.data && T[2].pout.data &&
+                     !pfields_overlap( T[0].pout, T[1].pout, n, fp64 ? 8 : 4 ) &&
+                     !pfields_overlap( T[0].pout, T[2].pout, n, fp64 ? 8 : 4 ) &&
+                     !pfields_overlap( T[1].pout, T[2].pout, n, fp64 ? 8 : 4 );
     if ( num_terms == 1 || mpm )
     {
         a.nterms = num_terms;

@@ -36,25 +36,34 @@ struct Spline1D
     Real h[K];
 };
 
-// Evaluates Spline<ORDER>::stencil/value/gradient (Cabana_Grid_Splines.hpp
-// <1> :163-213, <2> :264-324, <3> :383-451) from the FP64 logical coordinate.
-// The fractional offset x0 - int(x0) is formed in FP64 and only then rounded
-// to Real, so the FP32 path keeps FP64 cell indices and offsets.  Returns false
-// when the stencil leaves [0, num_node) or the coordinate is not finite.
-template <int ORDER, class Real, bool NEED_H>
-__device__ __forceinline__ bool eval_spline_1d( double x0, Real rdx,
-                                                int num_node,
-                                                Spline1D<ORDER, Real>& s )
+// Stencil base node and fractional offset of a logical coordinate
+// (Spline<ORDER>::stencil, Cabana_Grid_Splines.hpp <1> :163-168, <2> :264-270,
+// <3> :383-390).  The offset x0 - int(x0) is formed in FP64 and only then
+// rounded to Real, so the FP32 path keeps FP64 cell indices and offsets.
+// Returns false when the stencil leaves [0, num_node) or the coordinate is
+// not finite.
+template <int ORDER, class Real>
+__device__ __forceinline__ bool spline_frac( double x0, int num_node, int& base,
+                                             Real& f )
 {
     constexpr int K = ORDER + 1;
     // Reject NaN / out-of-range before the int conversion.
     if ( !( x0 >= 0.0 && x0 < (double)num_node ) )
         return false;
     const int i0 = (int)x0;
-    s.base = ( ORDER >= 2 ) ? i0 - 1 : i0;
-    if ( s.base < 0 || s.base + K > num_node )
+    base = ( ORDER >= 2 ) ? i0 - 1 : i0;
+    if ( base < 0 || base + K > num_node )
         return false;
-    const Real f = (Real)( x0 - (double)i0 );
+    f = (Real)( x0 - (double)i0 );
+    return true;
+}
+
+// Spline<ORDER>::value/gradient (Cabana_Grid_Splines.hpp <1> :170-213, <2>
+// :272-324, <3> :392-451) and second derivatives from the fractional offset.
+template <int ORDER, class Real, bool NEED_H>
+__device__ __forceinline__ void spline_weights( Real f, Real rdx,
+                                                Spline1D<ORDER, Real>& s )
+{
     if constexpr ( ORDER == 1 )
     {
         s.w[0] = Real( 1 ) - f;
@@ -121,6 +130,19 @@ __device__ __forceinline__ bool eval_spline_1d( double x0, Real rdx,
             s.h[3] = h3 * r2;
         }
     }
+}
+
+// Stencil base, weights, gradients (and second derivatives) of one
+// coordinate.
+template <int ORDER, class Real, bool NEED_H>
+__device__ __forceinline__ bool eval_spline_1d( double x0, Real rdx,
+                                                int num_node,
+                                                Spline1D<ORDER, Real>& s )
+{
+    Real f;
+    if ( !spline_frac<ORDER, Real>( x0, num_node, s.base, f ) )
+        return false;
+    spline_weights<ORDER, Real, NEED_H>( f, rdx, s );
     return true;
 }
 
@@ -450,6 +472,10 @@ __device__ __forceinline__ void eval_jets( const Spline3D<ORDER, Real, SH>& s,
                         Th[c] += s.d[2].h[k] * f[c];
                 }
             }
+            // One stencil row per scheduling region: the row's loads are
+            // not hoisted into the previous row, which bounds the live
+            // values to one row.
+            __builtin_amdgcn_sched_barrier( 0 );
             const Real wy = s.d[1].w[j];
 #pragma unroll
             for ( int c = 0; c < NG; ++c )
@@ -603,8 +629,22 @@ __device__ __forceinline__ void static_for( F&& f )
     static_for_impl( f, std::make_index_sequence<N>{} );
 }
 
+// Gather pass modes: outputs only (G2P forward), outputs and position
+// cotangent (P2G adjoint), position cotangent only (position part of the
+// G2P adjoint, or a P2G adjoint without value cotangents).
+enum GatherMode : int
+{
+    GATHER_OUT = 0,
+    GATHER_OUT_XBAR = 1,
+    GATHER_XBAR = 2
+};
+constexpr bool mode_xbar( int mode ) { return mode != GATHER_OUT; }
+constexpr bool mode_out( int mode ) { return mode != GATHER_XBAR; }
+
 // Jet levels every grid slot of a gather pass needs (SHARED: one slot read
-// by every term).
+// by every term): values for value-type outputs, gradients for gradient-type
+// outputs and value-type position cotangents, Hessians for gradient-type
+// position cotangents.
 template <int... FORMS>
 struct GatherNeeds
 {
@@ -614,25 +654,26 @@ struct GatherNeeds
     {
         return shared || t == slot;
     }
-    static constexpr bool V( int slot, bool shared )
+    static constexpr bool V( int slot, bool shared, int mode )
     {
         for ( int t = 0; t < N; ++t )
-            if ( uses( t, slot, shared ) && form_value_type( f[t] ) )
+            if ( uses( t, slot, shared ) && form_value_type( f[t] ) && mode_out( mode ) )
                 return true;
         return false;
     }
-    static constexpr bool G( int slot, bool shared, bool xbar )
+    static constexpr bool G( int slot, bool shared, int mode )
     {
         for ( int t = 0; t < N; ++t )
             if ( uses( t, slot, shared ) &&
-                 ( !form_value_type( f[t] ) || xbar ) )
+                 ( ( !form_value_type( f[t] ) && mode_out( mode ) ) ||
+                   ( form_value_type( f[t] ) && mode_xbar( mode ) ) ) )
                 return true;
         return false;
     }
-    static constexpr bool H( int slot, bool shared, bool xbar )
+    static constexpr bool H( int slot, bool shared, int mode )
     {
         for ( int t = 0; t < N; ++t )
-            if ( uses( t, slot, shared ) && !form_value_type( f[t] ) && xbar )
+            if ( uses( t, slot, shared ) && !form_value_type( f[t] ) && mode_xbar( mode ) )
                 return true;
         return false;
     }
@@ -643,66 +684,98 @@ struct GatherNeeds
     }
 };
 
-// One particle of a gather pass (G2P forward, P2G adjoint, or the position
-// part of the G2P adjoint): jets of every grid slot, then for each term
-//   pout += mult * dB/dP      and      xbar += mult * dB/dx.
+// One particle of a gather pass: for each grid slot, its jets, then for
+// each term reading that slot
+//   pout += mult * dB/dP      (MODE has outputs)
+//   xbar += mult * dB/dx      (MODE has the position cotangent).
+// Slots are finished one after the other, so only one slot's jets are live.
 // Load(slot_constant, i, j, k, f) fills the slot's NG values of stencil node
 // (i, j, k).
-template <int ORDER, class Real, bool XBAR, bool SHARED, int... FORMS,
-          class Load>
+template <int ORDER, class Real, int MODE, bool SHARED, int... FORMS, class Load>
 __device__ __forceinline__ void gather_particle(
-    const GatherArgs& a, int64_t p, const Spline3D<ORDER, Real, XBAR>& s,
-    Load&& load )
+    const GatherArgs& a, int64_t p,
+    const Spline3D<ORDER, Real, mode_xbar( MODE )>& s, Load&& load )
 {
     using GN = GatherNeeds<FORMS...>;
     constexpr int NT = GN::N;
     constexpr int NSLOT = SHARED ? 1 : NT;
-    Jet<Real> J[NSLOT][3];
-    static_for<NSLOT>( [&]( auto S ) {
-        constexpr int sl = S;
-        constexpr int NG = GN::NG( sl, SHARED );
-        eval_jets<ORDER, NG, GN::V( sl, SHARED ), GN::G( sl, SHARED, XBAR ),
-                  GN::H( sl, SHARED, XBAR )>(
-            s, [&]( int i, int j, int k, Real* fv ) { load( S, i, j, k, fv ); },
-            J[sl] );
-    } );
-    double xb[3] = { 0.0, 0.0, 0.0 };
+    constexpr bool XBAR = mode_xbar( MODE );
+    constexpr bool OUT = mode_out( MODE );
+    // Every global read of the particle (accumulated outputs, attributes of
+    // the position adjoint, the position cotangent) is issued before the
+    // stencil work, so its latency overlaps the jets.
+    Real old[NT][9];
+    Real P[NT][9];
     static_for<NT>( [&]( auto I ) {
         constexpr int t = I;
-        constexpr int F = GN::f[t];
-        constexpr int NP = FormTraits<F>::NP;
-        const Jet<Real>* Jt = J[SHARED ? 0 : t];
-        const Real m = (Real)a.t[t].mult;
-        const PFieldDev& o = a.t[t].pout;
-        if ( o.data )
+        constexpr int NP = FormTraits<GN::f[t]>::NP;
+        if constexpr ( OUT )
         {
-            Real out[NP];
-            jet_dP<F>( Jt, out );
-            const int64_t b = o.base( p );
+            const PFieldDev& o = a.t[t].pout;
+            const uint32_t b = o.base( p );
 #pragma unroll
             for ( int c = 0; c < NP; ++c )
-                *o.template ptr<Real>( b, c ) += m * out[c];
+                old[t][c] = *o.template ptr<const Real>( b, c );
         }
         if constexpr ( XBAR )
         {
-            Real P[NP];
-            const int64_t b = a.t[t].px.base( p );
+            const uint32_t b = a.t[t].px.base( p );
 #pragma unroll
             for ( int c = 0; c < NP; ++c )
-                P[c] = *a.t[t].px.template ptr<const Real>( b, c );
-            Real xt[3] = { Real( 0 ), Real( 0 ), Real( 0 ) };
-            jet_dx<F>( P, Jt, xt );
-#pragma unroll
-            for ( int e = 0; e < 3; ++e )
-                xb[e] += (double)( m * xt[e] );
+                P[t][c] = *a.t[t].px.template ptr<const Real>( b, c );
         }
+    } );
+    double xb[3] = { 0.0, 0.0, 0.0 };
+    if constexpr ( XBAR )
+    {
+        const uint32_t b = a.xbar.base( p );
+#pragma unroll
+        for ( int e = 0; e < 3; ++e )
+            xb[e] = *a.xbar.ptr<const double>( b, e );
+    }
+    static_for<NSLOT>( [&]( auto S ) {
+        constexpr int sl = S;
+        constexpr int NG = GN::NG( sl, SHARED );
+        Jet<Real> J[3];
+        eval_jets<ORDER, NG, GN::V( sl, SHARED, MODE ), GN::G( sl, SHARED, MODE ),
+                  GN::H( sl, SHARED, MODE )>(
+            s, [&]( int i, int j, int k, Real* fv ) { load( S, i, j, k, fv ); }, J );
+        static_for<NT>( [&]( auto I ) {
+            constexpr int t = I;
+            if constexpr ( SHARED || t == sl )
+            {
+                constexpr int F = GN::f[t];
+                constexpr int NP = FormTraits<F>::NP;
+                const Real m = (Real)a.t[t].mult;
+                if constexpr ( OUT )
+                {
+                    const PFieldDev& o = a.t[t].pout;
+                    Real out[NP];
+                    jet_dP<F>( J, out );
+                    const uint32_t b = o.base( p );
+#pragma unroll
+                    for ( int c = 0; c < NP; ++c )
+                        *o.template ptr<Real>( b, c ) = old[t][c] + m * out[c];
+                }
+                if constexpr ( XBAR )
+                {
+                    Real xt[3] = { Real( 0 ), Real( 0 ), Real( 0 ) };
+                    jet_dx<F>( P[t], J, xt );
+#pragma unroll
+                    for ( int e = 0; e < 3; ++e )
+                        xb[e] += (double)( m * xt[e] );
+                }
+            }
+        } );
+        // Finish this slot before the next slot's loads start.
+        __builtin_amdgcn_sched_barrier( 0 );
     } );
     if constexpr ( XBAR )
     {
-        const int64_t b = a.xbar.base( p );
+        const uint32_t b = a.xbar.base( p );
 #pragma unroll
         for ( int e = 0; e < 3; ++e )
-            *a.xbar.ptr<double>( b, e ) += xb[e];
+            *a.xbar.ptr<double>( b, e ) = xb[e];
     }
 }
 
@@ -736,7 +809,7 @@ __device__ __forceinline__ void atomic_add( Real* p, Real v )
 __device__ __forceinline__ void load_position( const PFieldDev& f, int64_t p,
                                                double px[3] )
 {
-    const int64_t b = f.base( p );
+    const uint32_t b = f.base( p );
     px[0] = *f.ptr<double>( b, 0 );
     px[1] = *f.ptr<double>( b, 1 );
     px[2] = *f.ptr<double>( b, 2 );

@@ -194,7 +194,7 @@ __global__ __launch_bounds__( BLOCK ) void lcl_count_kernel(
     const int64_t p = begin + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if ( p >= end )
         return;
-    const int64_t b = pos.base( p );
+    const uint32_t b = pos.base( p );
     const double x[3] = { *pos.ptr<const double>( b, 0 ),
                           *pos.ptr<const double>( b, 1 ),
                           *pos.ptr<const double>( b, 2 ) };
@@ -348,7 +348,7 @@ __global__ __launch_bounds__( BLOCK ) void permute_gather_kernel(
     if ( i >= end )
         return;
     const int64_t src = permutes[i - begin];
-    const int64_t b = f.base( src );
+    const uint32_t b = f.base( src );
     for ( int c = 0; c < f.ncomp; ++c )
         scratch[( i - begin ) * f.ncomp + c] = *f.ptr<const T>( b, c );
 }
@@ -360,9 +360,52 @@ __global__ __launch_bounds__( BLOCK ) void permute_copy_back_kernel(
     const int64_t i = begin + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if ( i >= end )
         return;
-    const int64_t b = f.base( i );
+    const uint32_t b = f.base( i );
     for ( int c = 0; c < f.ncomp; ++c )
         *f.ptr<T>( b, c ) = scratch[( i - begin ) * f.ncomp + c];
+}
+
+// Part table of a brick tiling.  One thread per brick counts the brick's
+// particles through the cell offsets of its cell runs and the number of
+// parts of at most t.cap particles (an empty brick has none).
+__global__ __launch_bounds__( BLOCK ) void brick_count_kernel( TileArgs t, int3 nc,
+                                                               int32_t* nparts )
+{
+    const int nbricks = t.ntile[0] * t.ntile[1] * t.ntile[2];
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    if ( b >= nbricks )
+        return;
+    const int bi = b / ( t.ntile[1] * t.ntile[2] );
+    const int bj = ( b / t.ntile[2] ) % t.ntile[1];
+    const int bk = b % t.ntile[2];
+    const int i0 = bi * t.tile[0], i1 = min( i0 + t.tile[0], nc.x );
+    const int j0 = bj * t.tile[1], j1 = min( j0 + t.tile[1], nc.y );
+    const int k0 = bk * t.tile[2], k1 = min( k0 + t.tile[2], nc.z );
+    int64_t count = 0;
+    for ( int i = i0; i < i1; ++i )
+        for ( int j = j0; j < j1; ++j )
+        {
+            const int64_t first = ( (int64_t)i * nc.y + j ) * nc.z + k0;
+            const int64_t last = first + ( k1 - k0 ) - 1;
+            count += t.cell_offsets[last] + t.cell_counts[last] - t.cell_offsets[first];
+        }
+    nparts[b] = (int32_t)( ( count + t.cap - 1 ) / t.cap );
+}
+
+__global__ __launch_bounds__( BLOCK ) void part_fill_kernel( const int32_t* nparts,
+                                                             const int64_t* start,
+                                                             int nbricks, int2* parts,
+                                                             int32_t* num_parts )
+{
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    if ( b >= nbricks )
+        return;
+    const int64_t s0 = start[b];
+    const int np = nparts[b];
+    for ( int q = 0; q < np; ++q )
+        parts[s0 + q] = make_int2( b, q );
+    if ( b == nbricks - 1 )
+        *num_parts = (int32_t)( s0 + np );
 }
 
 __global__ __launch_bounds__( BLOCK ) void fill_i32_kernel( int32_t* dst,
@@ -577,6 +620,31 @@ hipError_t launch_copy_i64( Launcher& L, const int64_t* src, int64_t* dst,
     hipLaunchKernelGGL( copy_i64_kernel, blocks_for( n ), dim3( BLOCK ), 0, s,
                         src, dst, n );
     L.after( "copy_i64", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_parts( Launcher& L, const TileArgs& t,
+                               const int32_t num_cell[3], int32_t* brick_nparts,
+                               int64_t* brick_start, int64_t* block_sums,
+                               int64_t nblock_cap, int2* parts,
+                               int32_t* num_parts, hipStream_t s )
+{
+    const int64_t nbricks = (int64_t)t.ntile[0] * t.ntile[1] * t.ntile[2];
+    if ( nbricks <= 0 )
+        return hipErrorInvalidValue;
+    L.before( "brick_parts", s );
+    hipLaunchKernelGGL( brick_count_kernel, blocks_for( nbricks ), dim3( BLOCK ), 0, s,
+                        t, make_int3( num_cell[0], num_cell[1], num_cell[2] ),
+                        brick_nparts );
+    L.after( "brick_parts", s );
+    hipError_t e = launch_exclusive_scan( L, brick_nparts, brick_start, nbricks,
+                                          block_sums, nblock_cap, s );
+    if ( e != hipSuccess )
+        return e;
+    L.before( "brick_parts", s );
+    hipLaunchKernelGGL( part_fill_kernel, blocks_for( nbricks ), dim3( BLOCK ), 0, s,
+                        brick_nparts, brick_start, (int)nbricks, parts, num_parts );
+    L.after( "brick_parts", s );
     return hipGetLastError();
 }
 

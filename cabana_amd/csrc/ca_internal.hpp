@@ -28,25 +28,30 @@ struct MeshDev
 // Particle field addressing (Cabana_Slice.hpp:119-190).  For a power-of-two
 // vector length the block and lane come from shift/mask; when one block holds
 // every particle, shift = 62 so the block index is always 0.
+// Particle field (Cabana slice addressing, Cabana_Slice.hpp:119-190):
+// element (p, c) at data[(p / vl) * stride + p % vl + vl * c].  The vector
+// length is a power of two (as every Cabana AoSoA's) or covers all particles,
+// so the index is a shift and a mask; every element index of the field fits
+// in 32 bits.  Both are checked on the host; device addressing is 32-bit
+// integer math plus one 64-bit pointer add, with no branches.
 struct PFieldDev
 {
     char* data;
-    int64_t vl;
-    int64_t stride;
-    int32_t shift; // -1: general vector length (division)
+    uint32_t vl;
+    uint32_t stride;
+    int32_t shift;
     int32_t ncomp;
-    int64_t mask;
+    uint32_t mask;
 
-    __device__ __forceinline__ int64_t base( int64_t p ) const
+    __device__ __forceinline__ uint32_t base( int64_t p64 ) const
     {
-        if ( shift >= 0 )
-            return ( p >> shift ) * stride + ( p & mask );
-        return ( p / vl ) * stride + ( p % vl );
+        const uint32_t p = (uint32_t)p64;
+        return ( p >> shift ) * stride + ( p & mask );
     }
     template <class T>
-    __device__ __forceinline__ T* ptr( int64_t b, int c ) const
+    __device__ __forceinline__ T* ptr( uint32_t b, int c ) const
     {
-        return reinterpret_cast<T*>( data ) + b + vl * c;
+        return reinterpret_cast<T*>( data + (uint64_t)( b + vl * (uint32_t)c ) * sizeof( T ) );
     }
 };
 
@@ -159,23 +164,44 @@ struct GatherArgs
     unsigned long long* skipped;
 };
 
-// Tile description for the cell-sorted scatter: a brick of cells whose
-// particles are read through cell offsets.
+// Brick tiling of cell-sorted particles.  A brick is tile[0] x tile[1] x
+// tile[2] mesh cells; its particles are the tile[0] * tile[1] cell runs along
+// the last dimension (contiguous in the cell-sorted order).  Bricks are cut
+// into parts of at most `cap` particles; a workgroup processes one part at a
+// time from the part table (filled on the device before the launch).
 struct TileArgs
 {
     const int64_t* cell_offsets; // exclusive scan, per mesh cell
     const int32_t* cell_counts;
-    int32_t tile[3];             // cells per tile and dimension
-    int32_t ntile[3];
+    const int2* parts;           // (brick, part index within the brick)
+    const int32_t* num_parts;    // number of entries of `parts`
+    int32_t tile[3];             // cells per brick and dimension
+    int32_t ntile[3];            // bricks per dimension
     int32_t win[3];              // window nodes per dimension
-    int32_t tiles_per_xcd;       // blocks are dealt round-robin to 8 XCDs
     int32_t flush_perm[3];       // grid dims from largest to smallest stride
     int32_t max_runs;            // tile[0] * tile[1]
     int32_t max_win_nodes;       // win[0] * win[1] * win[2]
-    int32_t max_base_cells;      // stencil-base cells of an unclipped tile
-    int32_t cap;                 // particles sorted per batch (LDS capacity)
+    int32_t max_base_cells;      // stencil-base cells of a brick
+    int32_t cap;                 // particles per part
+    int32_t sub_len;             // scatter: particles per register task
+    int32_t max_tasks;           // scatter: task-table capacity
+    int32_t grid;                // workgroups launched (a multiple of 8)
     int64_t bin_begin;           // particle index of cell offset 0
 };
+
+// LDS record of one particle in the cell-sorted scatter: 3 fractional
+// offsets then the premultiplied attributes.  The record stride (in
+// elements) keeps 16-byte alignment and avoids a stride that is a multiple
+// of 32 dwords, so concurrent lanes spread over the LDS banks.
+__host__ __device__ constexpr int record_stride( int len, int elem_bytes )
+{
+    int dw = ( len * elem_bytes + 15 ) / 16 * 4;
+    if ( dw % 8 == 0 )
+        dw += 4;
+    return dw * 4 / elem_bytes;
+}
+// Largest part the scatter sorts in LDS (bounded by its register key array).
+constexpr int SCATTER_CAP_MAX = 768;
 
 // Launch-time error/timing context owned by the C-ABI layer.
 class Launcher
@@ -267,5 +293,13 @@ hipError_t launch_copy_i64( Launcher& L, const int64_t* src, int64_t* dst,
                             int64_t n, hipStream_t s );
 hipError_t launch_fill_i32( Launcher& L, int32_t* dst, int64_t n,
                             int32_t value, hipStream_t s );
+// Part table of a brick tiling: brick_nparts (nbricks ints), brick_start
+// (nbricks int64), block_sums (scan scratch, nblock_cap int64), parts
+// (nbricks + n / cap entries at most) and num_parts (1 int).
+hipError_t launch_brick_parts( Launcher& L, const TileArgs& t,
+                               const int32_t num_cell[3], int32_t* brick_nparts,
+                               int64_t* brick_start, int64_t* block_sums,
+                               int64_t nblock_cap, int2* parts,
+                               int32_t* num_parts, hipStream_t s );
 
 } // namespace cabana_amd

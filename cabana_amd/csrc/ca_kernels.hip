@@ -47,7 +47,7 @@ __device__ __forceinline__ const Real* grid_at( const GFieldDev& g, int i,
 // Gather: pout += mult * dB/dP, xbar += mult * dB/dx.
 // ---------------------------------------------------------------------------
 
-template <class Real, bool XBAR, bool SHARED, int... FORMS>
+template <class Real, int MODE, bool SHARED, int... FORMS>
 __global__ __launch_bounds__( BLOCK ) void gather_kernel(
     GatherArgs a )
 {
@@ -56,7 +56,7 @@ __global__ __launch_bounds__( BLOCK ) void gather_kernel(
         return;
     double px[3];
     load_position( a.points, p, px );
-    Spline3D<ORDER, Real, XBAR> s;
+    Spline3D<ORDER, Real, mode_xbar( MODE )> s;
     if ( !eval_spline( a.mesh, px, s ) )
     {
         atomicAdd( a.skipped, 1ull );
@@ -64,7 +64,7 @@ __global__ __launch_bounds__( BLOCK ) void gather_kernel(
     }
     GlobalGatherLoad<Real, SHARED, FORMS...> ld{
         a, { s.d[0].base, s.d[1].base, s.d[2].base } };
-    gather_particle<ORDER, Real, XBAR, SHARED, FORMS...>( a, p, s, ld );
+    gather_particle<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, ld );
 }
 
 // ---------------------------------------------------------------------------
@@ -88,7 +88,7 @@ __device__ __forceinline__ void scatter_particle( const ScatterArgs& a,
         constexpr int t = I;
         constexpr int NP = FormTraits<FL::v[t]>::NP;
         const Real m = (Real)a.t[t].mult;
-        const int64_t b = a.t[t].pin.base( p );
+        const uint32_t b = a.t[t].pin.base( p );
 #pragma unroll
         for ( int c = 0; c < NP; ++c )
         {
@@ -170,7 +170,7 @@ __device__ __forceinline__ void scatter_particle( const ScatterArgs& a,
             for ( int e = 0; e < 3; ++e )
                 xb[e] += (double)( m * xt[t][e] );
         } );
-        const int64_t b = a.xbar.base( p );
+        const uint32_t b = a.xbar.base( p );
 #pragma unroll
         for ( int e = 0; e < 3; ++e )
             *a.xbar.ptr<double>( b, e ) += xb[e];
@@ -228,36 +228,33 @@ inline dim3 blocks_for( int64_t n )
     return dim3( (unsigned)( ( n + BLOCK - 1 ) / BLOCK ) );
 }
 
-template <class Real, bool XBAR>
+// Gather mode of a pass: outputs and/or the position cotangent (a fused
+// pass has either every term's output or none).
+inline int gather_mode( const GatherArgs& a )
+{
+    if ( !a.has_xbar )
+        return GATHER_OUT;
+    return a.t[0].pout.data ? GATHER_OUT_XBAR : GATHER_XBAR;
+}
+
+template <class Real, int MODE>
 hipError_t gather_single( const GatherArgs& a, hipStream_t s )
 {
     const dim3 g = blocks_for( a.end - a.begin );
     switch ( a.t[0].form )
     {
-    case FORM_SV:
-        hipLaunchKernelGGL( ( gather_kernel<Real, XBAR, false, FORM_SV> ), g,
-                            dim3( BLOCK ), 0, s, a );
+#define CA_CASE( F )                                                           \
+    case F:                                                                    \
+        hipLaunchKernelGGL( ( gather_kernel<Real, MODE, false, F> ), g, dim3( BLOCK ), 0, \
+                            s, a );                                            \
         break;
-    case FORM_VV:
-        hipLaunchKernelGGL( ( gather_kernel<Real, XBAR, false, FORM_VV> ), g,
-                            dim3( BLOCK ), 0, s, a );
-        break;
-    case FORM_SG:
-        hipLaunchKernelGGL( ( gather_kernel<Real, XBAR, false, FORM_SG> ), g,
-                            dim3( BLOCK ), 0, s, a );
-        break;
-    case FORM_VD:
-        hipLaunchKernelGGL( ( gather_kernel<Real, XBAR, false, FORM_VD> ), g,
-                            dim3( BLOCK ), 0, s, a );
-        break;
-    case FORM_TD:
-        hipLaunchKernelGGL( ( gather_kernel<Real, XBAR, false, FORM_TD> ), g,
-                            dim3( BLOCK ), 0, s, a );
-        break;
-    case FORM_TDT:
-        hipLaunchKernelGGL( ( gather_kernel<Real, XBAR, false, FORM_TDT> ), g,
-                            dim3( BLOCK ), 0, s, a );
-        break;
+        CA_CASE( FORM_SV )
+        CA_CASE( FORM_VV )
+        CA_CASE( FORM_SG )
+        CA_CASE( FORM_VD )
+        CA_CASE( FORM_TD )
+        CA_CASE( FORM_TDT )
+#undef CA_CASE
     default:
         return hipErrorInvalidValue;
     }
@@ -272,34 +269,36 @@ hipError_t gather_typed( Launcher& L, int plan, const GatherArgs& a,
         return hipSuccess;
     const dim3 g = blocks_for( a.end - a.begin );
     hipError_t e = hipSuccess;
+    const int mode = gather_mode( a );
     const char* name =
         a.has_xbar ? ( plan == PLAN_MPM_P2G ? "p2g_adjoint_mpm" : "p2g_adjoint" )
                    : ( plan == PLAN_MPM_G2P ? "g2p_mpm" : "g2p" );
     L.before( name, s );
     if ( plan == PLAN_SINGLE )
     {
-        e = a.has_xbar ? gather_single<Real, true>( a, s )
-                       : gather_single<Real, false>( a, s );
+        e = mode == GATHER_OUT        ? gather_single<Real, GATHER_OUT>( a, s )
+            : mode == GATHER_OUT_XBAR ? gather_single<Real, GATHER_OUT_XBAR>( a, s )
+                                      : gather_single<Real, GATHER_XBAR>( a, s );
     }
-    else if ( plan == PLAN_MPM_G2P && !a.has_xbar )
+    else if ( plan == PLAN_MPM_G2P && mode == GATHER_OUT )
     {
         hipLaunchKernelGGL(
-            ( gather_kernel<Real, false, true, FORM_VV, FORM_TDT> ), g,
+            ( gather_kernel<Real, GATHER_OUT, true, FORM_VV, FORM_TDT> ), g,
             dim3( BLOCK ), 0, s, a );
         e = hipGetLastError();
     }
-    else if ( plan == PLAN_MPM_G2P && a.has_xbar )
+    else if ( plan == PLAN_MPM_G2P && mode == GATHER_XBAR )
     {
         hipLaunchKernelGGL(
-            ( gather_kernel<Real, true, true, FORM_VV, FORM_TDT> ), g,
+            ( gather_kernel<Real, GATHER_XBAR, true, FORM_VV, FORM_TDT> ), g,
             dim3( BLOCK ), 0, s, a );
         e = hipGetLastError();
     }
-    else if ( plan == PLAN_MPM_P2G && a.has_xbar )
+    else if ( plan == PLAN_MPM_P2G && mode == GATHER_OUT_XBAR )
     {
         hipLaunchKernelGGL(
-            ( gather_kernel<Real, true, false, FORM_SV, FORM_VV, FORM_TD> ), g,
-            dim3( BLOCK ), 0, s, a );
+            ( gather_kernel<Real, GATHER_OUT_XBAR, false, FORM_SV, FORM_VV, FORM_TD> ),
+            g, dim3( BLOCK ), 0, s, a );
         e = hipGetLastError();
     }
     else

@@ -1,24 +1,30 @@
 // Cell-tiled interpolation kernels for cell-sorted particles, one spline
 // order per translation unit (CA_ORDER set on the compile line).
 //
-// Both kernels give one workgroup a brick of T^3 mesh cells.  The brick's
-// particles are the T^2 k-runs of the cell-sorted list (cell offsets of the
-// binning), walked through a flattened run index so consecutive threads read
-// consecutive particles.
+// Bricks of mesh cells are cut into parts of bounded particle count (a part
+// table built on the device before the launch), and each workgroup walks
+// parts from an XCD-local range of that table, so dense clusters are spread
+// over every CU and empty bricks cost nothing.  A brick's particles are its
+// cell runs along the last dimension (contiguous in the cell-sorted order),
+// walked through a flattened index so consecutive lanes read consecutive
+// particles.
 //
 //   gather_tiled_kernel   G2P forward, P2G adjoint and the position part of
 //                         the G2P adjoint.  The node window of every grid the
-//                         pass reads ((T + K - 1 + shift)^3 nodes) is staged
-//                         in LDS once with coalesced loads; every particle
-//                         then gathers its K^3 stencil from LDS.
+//                         pass reads is staged in LDS with coalesced loads;
+//                         every particle then gathers its K^3 stencil from
+//                         LDS.
 //   scatter_cells_kernel  P2G forward and the grid part of the G2P adjoint.
-//                         The brick's particles are counting-sorted in LDS by
-//                         their stencil base node; one lane owns one
-//                         (base cell, x-slab) pair, accumulates the slab's
-//                         K^2 nodes in registers over all particles of the
-//                         base cell, and flushes once into an FP64 LDS window
-//                         (LDS FP64 adds are ~16x faster than FP32 ones on
-//                         gfx950).  The window is flushed into the grid once.
+//                         The part's particles are counting-sorted in LDS by
+//                         stencil base cell and stored there as compact
+//                         records (FP32/FP64 cell offsets + premultiplied
+//                         attributes, loaded with coalesced reads).  One lane
+//                         then owns a run of at most sub_len particles of one
+//                         base cell, accumulates their whole stencil in
+//                         registers from the LDS records only, and adds it
+//                         once into an FP64 LDS window (LDS FP64 adds are
+//                         ~16x faster than FP32 ones on gfx950).  The window
+//                         is flushed into the grid once per part.
 #include "ca_device.hpp"
 
 #include <utility>
@@ -72,31 +78,48 @@ __device__ __forceinline__ size_t align16( size_t b )
 
 struct TileGeom
 {
-    int tile;
+    int brick;
+    int part;
     int c0[3];
     int c1[3];
 };
 
-__device__ __forceinline__ bool tile_geometry( const TileArgs& ta,
-                                               const MeshDev& m, TileGeom& g )
+// The parts of this workgroup: the part table is split into 8 contiguous
+// ranges, one per XCD (workgroups are dealt round-robin to the XCDs), so
+// neighbouring bricks share an L2; the workgroups of one XCD stride through
+// its range.  Every workgroup leaves the loop once its range is exhausted.
+struct PartLoop
 {
-    // Blocks are dealt round-robin to the 8 XCDs: give each XCD a
-    // contiguous range of tiles so neighbouring windows share an L2.
-    const int b = blockIdx.x;
-    g.tile = ( b & 7 ) * ta.tiles_per_xcd + ( b >> 3 );
-    const int ntiles = ta.ntile[0] * ta.ntile[1] * ta.ntile[2];
-    if ( g.tile >= ntiles )
-        return false;
-    const int ti = g.tile / ( ta.ntile[1] * ta.ntile[2] );
-    const int tj = ( g.tile / ta.ntile[2] ) % ta.ntile[1];
-    const int tk = g.tile % ta.ntile[2];
+    int total;
+    int per_xcd;
+    int xcd;
+    int first;
+    int stride;
+    __device__ explicit PartLoop( const TileArgs& ta )
+    {
+        total = *ta.num_parts;
+        per_xcd = ( total + 7 ) >> 3;
+        xcd = blockIdx.x & 7;
+        first = blockIdx.x >> 3;
+        stride = gridDim.x >> 3;
+    }
+};
+
+__device__ __forceinline__ void part_geometry( const TileArgs& ta, const MeshDev& m,
+                                               int w, TileGeom& g )
+{
+    const int2 pr = ta.parts[w];
+    g.brick = pr.x;
+    g.part = pr.y;
+    const int ti = g.brick / ( ta.ntile[1] * ta.ntile[2] );
+    const int tj = ( g.brick / ta.ntile[2] ) % ta.ntile[1];
+    const int tk = g.brick % ta.ntile[2];
     g.c0[0] = ti * ta.tile[0];
     g.c0[1] = tj * ta.tile[1];
     g.c0[2] = tk * ta.tile[2];
 #pragma unroll
     for ( int d = 0; d < 3; ++d )
         g.c1[d] = min( g.c0[d] + ta.tile[d], m.num_cell[d] );
-    return true;
 }
 
 // Fills the run table of the brick (one run per (i, j) cell row) and returns
@@ -147,6 +170,21 @@ __device__ __forceinline__ int64_t run_particle( const TileArgs& ta,
     return ta.bin_begin + run_start[lo] + ( v - run_pre[lo] );
 }
 
+// Run containing flattened brick position v (binary search).
+__device__ __forceinline__ int find_run( const int* run_pre, int nruns, int v )
+{
+    int lo = 0, hi = nruns - 1;
+    while ( lo < hi )
+    {
+        const int mid = ( lo + hi + 1 ) >> 1;
+        if ( run_pre[mid] <= v )
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // Maps a flat window index to (node, dof) with the grid's unit-stride
 // dimension fastest, so window loads/stores coalesce.
 __device__ __forceinline__ void window_coords( const TileArgs& ta, int f,
@@ -166,7 +204,7 @@ __device__ __forceinline__ void window_coords( const TileArgs& ta, int f,
 // Tiled gather
 // ---------------------------------------------------------------------------
 
-template <class Real, bool XBAR, bool SHARED, int... FORMS>
+template <class Real, int MODE, bool SHARED, int... FORMS>
 __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
     GatherArgs a, TileArgs ta )
 {
@@ -175,19 +213,7 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
     constexpr int NSLOT = SHARED ? 1 : NT;
     extern __shared__ __align__( 16 ) unsigned char smem[];
 
-    TileGeom tg;
-    if ( !tile_geometry( ta, a.mesh, tg ) )
-        return;
-    int wlo[3], W[3];
-#pragma unroll
-    for ( int d = 0; d < 3; ++d )
-    {
-        wlo[d] = tg.c0[d] + a.mesh.halo + BASE_SHIFT;
-        W[d] = ta.win[d];
-    }
-    const int wnodes = W[0] * W[1] * W[2];
-
-    // LDS: staged windows, then the run table.
+    // LDS: staged windows, then the run table (capacities of a full brick).
     Real* win[NT];
     size_t off = 0;
     static_for<NT>( [&]( auto I ) {
@@ -195,7 +221,7 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
         if ( t < NSLOT )
         {
             win[t] = reinterpret_cast<Real*>( smem + off );
-            off = align16( off + sizeof( Real ) * wnodes * ngp_of( FL::v[t] ) );
+            off = align16( off + sizeof( Real ) * ta.max_win_nodes * ngp_of( FL::v[t] ) );
         }
         else
         {
@@ -205,79 +231,130 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
     int64_t* run_start = reinterpret_cast<int64_t*>( smem + off );
     int* run_pre = reinterpret_cast<int*>( run_start + ta.max_runs );
 
-    // Stage the windows (nodes outside the grid read as zero).
-    static_for<NSLOT>( [&]( auto I ) {
-        constexpr int t = I;
-        constexpr int NG = ng_of( FL::v[t] );
-        constexpr int NGP = ngp_of( FL::v[t] );
-        const GFieldDev& g = a.t[t].grid;
-        Real* w = win[t];
-        for ( int f = threadIdx.x; f < wnodes * NG; f += BLOCK )
-        {
-            int q[3], d;
-            window_coords( ta, f, NG, W, q, d );
-            const int gi = wlo[0] + q[0], gj = wlo[1] + q[1],
-                      gk = wlo[2] + q[2];
-            Real v = Real( 0 );
-            if ( gi >= 0 && gj >= 0 && gk >= 0 && gi < a.mesh.num_node[0] &&
-                 gj < a.mesh.num_node[1] && gk < a.mesh.num_node[2] )
-                v = *gptr<const Real>( g, gi * g.s[0] + gj * g.s[1] +
-                                              gk * g.s[2] + d * g.s[3] );
-            w[( ( q[0] * W[1] + q[1] ) * W[2] + q[2] ) * NGP + d] = v;
-        }
-    } );
-    const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
-    const int total = setup_runs( ta, a.mesh, tg, run_start, run_pre );
-
-    for ( int v = threadIdx.x; v < total; v += BLOCK )
+    const PartLoop pl( ta );
+    for ( int loc = pl.first; loc < pl.per_xcd; loc += pl.stride )
     {
-        const int64_t p = run_particle( ta, run_start, run_pre, nruns, v );
-        double px[3];
-        load_position( a.points, p, px );
-        Spline3D<ORDER, Real, XBAR> s;
-        if ( !eval_spline( a.mesh, px, s ) )
-        {
-            atomicAdd( a.skipped, 1ull );
-            continue;
-        }
-        int rel[3];
-        bool fits = true;
+        const int w = pl.xcd * pl.per_xcd + loc;
+        if ( w >= pl.total )
+            break;
+        TileGeom tg;
+        part_geometry( ta, a.mesh, w, tg );
+        int wlo[3], W[3];
 #pragma unroll
         for ( int d = 0; d < 3; ++d )
         {
-            rel[d] = s.d[d].base - wlo[d];
-            fits = fits && rel[d] >= 0 && rel[d] + K <= W[d];
+            wlo[d] = tg.c0[d] + a.mesh.halo + BASE_SHIFT;
+            W[d] = ta.win[d];
         }
-        if ( fits )
+        const int wnodes = W[0] * W[1] * W[2];
+
+        // Stage the windows (nodes outside the grid read as zero).
+        static_for<NSLOT>( [&]( auto I ) {
+            constexpr int t = I;
+            constexpr int NG = ng_of( FL::v[t] );
+            constexpr int NGP = ngp_of( FL::v[t] );
+            const GFieldDev& g = a.t[t].grid;
+            Real* wp = win[t];
+            for ( int f = threadIdx.x; f < wnodes * NG; f += BLOCK )
+            {
+                int q[3], d;
+                window_coords( ta, f, NG, W, q, d );
+                const int gi = wlo[0] + q[0], gj = wlo[1] + q[1], gk = wlo[2] + q[2];
+                Real v = Real( 0 );
+                if ( gi >= 0 && gj >= 0 && gk >= 0 && gi < a.mesh.num_node[0] &&
+                     gj < a.mesh.num_node[1] && gk < a.mesh.num_node[2] )
+                    v = *gptr<const Real>( g, gi * g.s[0] + gj * g.s[1] + gk * g.s[2] +
+                                                  d * g.s[3] );
+                wp[( ( q[0] * W[1] + q[1] ) * W[2] + q[2] ) * NGP + d] = v;
+            }
+        } );
+        const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
+        const int total = setup_runs( ta, a.mesh, tg, run_start, run_pre );
+        const int pbeg = tg.part * ta.cap;
+        const int pend = min( total, pbeg + ta.cap );
+
+        // Lanes stride through the part; the cell run of the next particle
+        // is found by walking forward, and its position is loaded one
+        // iteration ahead.
+        int v = pbeg + (int)threadIdx.x;
+        int r = 0;
+        int64_t p = 0;
+        double px[3] = { 0.0, 0.0, 0.0 };
+        if ( v < pend )
         {
-            const int nb = ( rel[0] * W[1] + rel[1] ) * W[2] + rel[2];
-            const int W1 = W[1], W2 = W[2];
-            auto ld = [&]( auto S, int i, int j, int k, Real* f ) {
-                constexpr int sl = decltype( S )::value;
-                constexpr int NG = GatherNeeds<FORMS...>::NG( sl, SHARED );
-                const int node = nb + ( i * W1 + j ) * W2 + k;
-                if constexpr ( NG == 1 )
+            r = find_run( run_pre, nruns, v );
+            p = ta.bin_begin + run_start[r] + ( v - run_pre[r] );
+            load_position( a.points, p, px );
+        }
+        while ( v < pend )
+        {
+            const int vn = v + BLOCK;
+            int64_t pn = 0;
+            double pxn[3] = { 0.0, 0.0, 0.0 };
+            if ( vn < pend )
+            {
+                while ( run_pre[r + 1] <= vn )
+                    ++r;
+                pn = ta.bin_begin + run_start[r] + ( vn - run_pre[r] );
+                load_position( a.points, pn, pxn );
+            }
+            Spline3D<ORDER, Real, mode_xbar( MODE )> s;
+            if ( !eval_spline( a.mesh, px, s ) )
+            {
+                atomicAdd( a.skipped, 1ull );
+            }
+            else
+            {
+                int rel[3];
+                bool fits = true;
+#pragma unroll
+                for ( int d = 0; d < 3; ++d )
                 {
-                    f[0] = win[sl][node];
+                    rel[d] = s.d[d].base - wlo[d];
+                    fits = fits && rel[d] >= 0 && rel[d] + K <= W[d];
+                }
+                // Stencil nodes from the staged window.  A wave with a
+                // particle whose stencil is outside the window (not sorted
+                // into this brick) reads the grid directly for all of its
+                // lanes, so the choice is one uniform branch, not one per
+                // node.
+                if ( __all( fits ) )
+                {
+                    const int nb = ( rel[0] * W[1] + rel[1] ) * W[2] + rel[2];
+                    const int W1 = W[1], W2 = W[2];
+                    auto ld = [&]( auto S, int i, int j, int k, Real* f ) {
+                        constexpr int sl = decltype( S )::value;
+                        constexpr int NG = GatherNeeds<FORMS...>::NG( sl, SHARED );
+                        const int node = nb + ( i * W1 + j ) * W2 + k;
+                        if constexpr ( NG == 1 )
+                        {
+                            f[0] = win[sl][node];
+                        }
+                        else
+                        {
+                            const Real* q = win[sl] + node * 4;
+                            f[0] = q[0];
+                            f[1] = q[1];
+                            f[2] = q[2];
+                        }
+                    };
+                    gather_particle<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, ld );
                 }
                 else
                 {
-                    const Real* q = win[sl] + node * 4;
-                    f[0] = q[0];
-                    f[1] = q[1];
-                    f[2] = q[2];
+                    GlobalGatherLoad<Real, SHARED, FORMS...> ld{
+                        a, { s.d[0].base, s.d[1].base, s.d[2].base } };
+                    gather_particle<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, ld );
                 }
-            };
-            gather_particle<ORDER, Real, XBAR, SHARED, FORMS...>( a, p, s, ld );
+            }
+            v = vn;
+            p = pn;
+#pragma unroll
+            for ( int d = 0; d < 3; ++d )
+                px[d] = pxn[d];
         }
-        else
-        {
-            // Stencil outside the staged window (particle not sorted into
-            // this brick): read the grid directly.
-            GlobalGatherLoad<Real, SHARED, FORMS...> ld{
-                a, { s.d[0].base, s.d[1].base, s.d[2].base } };
-            gather_particle<ORDER, Real, XBAR, SHARED, FORMS...>( a, p, s, ld );
-        }
+        // The next part restages the windows.
+        __syncthreads();
     }
 }
 
@@ -322,15 +399,6 @@ __device__ __forceinline__ int block_exclusive_scan( int* v, int n, int* wsum )
     return total;
 }
 
-template <class Real>
-__device__ __forceinline__ Real pick( const Real* v, int q )
-{
-    Real r = v[0];
-#pragma unroll
-    for ( int i = 1; i < K; ++i )
-        r = ( q == i ) ? v[i] : r;
-    return r;
-}
 
 // Global-atomic scatter of one particle (particles whose stencil is outside
 // the brick's window).
@@ -346,7 +414,7 @@ __device__ __forceinline__ void global_scatter_particle( const ScatterArgs& a,
         constexpr int t = I;
         constexpr int NP = np_of( FL::v[t] );
         const Real m = (Real)a.t[t].mult;
-        const int64_t b = a.t[t].pin.base( p );
+        const uint32_t b = a.t[t].pin.base( p );
 #pragma unroll
         for ( int c = 0; c < NP; ++c )
             Pm[t][c] = *a.t[t].pin.template ptr<const Real>( b, c ) * m;
@@ -397,223 +465,331 @@ struct SlotDofs
     }
 };
 
+// Record layout of a scatter pass: 3 fractional offsets, then the
+// attributes of every term in order.
+template <int... FORMS>
+struct RecordLayout
+{
+    using FL = FormList<FORMS...>;
+    static constexpr int attr( int t )
+    {
+        int o = 3;
+        for ( int i = 0; i < t; ++i )
+            o += np_of( FL::v[i] );
+        return o;
+    }
+    static constexpr int len() { return attr( FL::N ); }
+};
+
+// Task-table entry: base cell (12 bits), first record (15 bits), count (5).
+__device__ __forceinline__ unsigned pack_task( int base, int start, int len )
+{
+    return ( (unsigned)base << 20 ) | ( (unsigned)start << 5 ) | (unsigned)len;
+}
+
 template <class Real, bool SHARED, int... FORMS>
 __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
     ScatterArgs a, TileArgs ta )
 {
     using FL = FormList<FORMS...>;
     using SD = SlotDofs<FORMS...>;
+    using RL = RecordLayout<FORMS...>;
     constexpr int NT = FL::N;
     constexpr int DT = SD::total( SHARED );
+    constexpr int RS = record_stride( RL::len(), (int)sizeof( Real ) );
+    constexpr int MAXM = SCATTER_CAP_MAX / BLOCK;
+    constexpr int NGROUP = SHARED ? 1 : NT;
     extern __shared__ __align__( 16 ) unsigned char smem[];
 
-    TileGeom tg;
-    if ( !tile_geometry( ta, a.mesh, tg ) )
-        return;
-    // Base cells of the brick and the node window they reach.
-    int blo[3], NB[3], W[3];
-#pragma unroll
-    for ( int d = 0; d < 3; ++d )
-    {
-        blo[d] = tg.c0[d] + a.mesh.halo + BASE_SHIFT;
-        NB[d] = ( tg.c1[d] - tg.c0[d] ) + BASE_SPREAD;
-        W[d] = ta.win[d];
-    }
-    const int nbt = NB[0] * NB[1] * NB[2];
-    const int wnodes = W[0] * W[1] * W[2];
-
-    // LDS layout (capacities fixed on the host from the unclipped tile):
-    // FP64 window [node][DT], base-cell offsets, batch keys, sorted list,
-    // run table, scan scratch.
+    // LDS layout (capacities fixed on the host for a full brick): FP64
+    // window [node][DT], base-cell offsets, task counts, task table,
+    // records, run table, scan scratch.
     double* win = reinterpret_cast<double*>( smem );
     size_t off = align16( sizeof( double ) * (size_t)DT * ta.max_win_nodes );
     int* cnt = reinterpret_cast<int*>( smem + off );
     off = align16( off + sizeof( int ) * ( ta.max_base_cells + 1 ) );
-    int* key = reinterpret_cast<int*>( smem + off );
-    off = align16( off + sizeof( int ) * ta.cap );
-    int* list = reinterpret_cast<int*>( smem + off );
-    off = align16( off + sizeof( int ) * ta.cap );
+    int* sub = reinterpret_cast<int*>( smem + off );
+    off = align16( off + sizeof( int ) * ( ta.max_base_cells + 1 ) );
+    unsigned* task = reinterpret_cast<unsigned*>( smem + off );
+    off = align16( off + sizeof( unsigned ) * ta.max_tasks );
+    Real* rec = reinterpret_cast<Real*>( smem + off );
+    off = align16( off + sizeof( Real ) * (size_t)RS * ta.cap );
     int64_t* run_start = reinterpret_cast<int64_t*>( smem + off );
     off = align16( off + sizeof( int64_t ) * ta.max_runs );
     int* run_pre = reinterpret_cast<int*>( smem + off );
     off = align16( off + sizeof( int ) * ( ta.max_runs + 1 ) );
     int* wsum = reinterpret_cast<int*>( smem + off );
 
-    for ( int e = threadIdx.x; e < DT * wnodes; e += BLOCK )
-        win[e] = 0.0;
-    const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
-    const int total = setup_runs( ta, a.mesh, tg, run_start, run_pre );
-
-    for ( int bstart = 0; bstart < total; bstart += ta.cap )
+    const Real rdx[3] = { (Real)a.mesh.rdx[0], (Real)a.mesh.rdx[1], (Real)a.mesh.rdx[2] };
+    const int L = ta.sub_len;
+    const PartLoop pl( ta );
+    for ( int loc = pl.first; loc < pl.per_xcd; loc += pl.stride )
     {
-        const int nbatch = min( ta.cap, total - bstart );
+        const int w = pl.xcd * pl.per_xcd + loc;
+        if ( w >= pl.total )
+            break;
+        TileGeom tg;
+        part_geometry( ta, a.mesh, w, tg );
+        // Base cells of the brick and the node window they reach.
+        int blo[3], NB[3], W[3];
+#pragma unroll
+        for ( int d = 0; d < 3; ++d )
+        {
+            blo[d] = tg.c0[d] + a.mesh.halo + BASE_SHIFT;
+            NB[d] = ( tg.c1[d] - tg.c0[d] ) + BASE_SPREAD;
+            W[d] = ta.win[d];
+        }
+        const int nbt = NB[0] * NB[1] * NB[2];
+        const int wnodes = W[0] * W[1] * W[2];
+        for ( int e = threadIdx.x; e < DT * wnodes; e += BLOCK )
+            win[e] = 0.0;
         for ( int c = threadIdx.x; c <= nbt; c += BLOCK )
             cnt[c] = 0;
-        __syncthreads();
-        // A: stencil base of every particle of the batch, counted per base
-        // cell (the slot is the particle's rank in its cell).
-        for ( int i = threadIdx.x; i < nbatch; i += BLOCK )
+        const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
+        const int total = setup_runs( ta, a.mesh, tg, run_start, run_pre );
+        const int pbeg = tg.part * ta.cap;
+        const int nbatch = min( ta.cap, total - pbeg );
+
+        // A: stencil base cell of every particle of the part, counted per
+        // base cell (the slot is the particle's rank within its base cell);
+        // the particle's record (offsets, premultiplied attributes) is kept
+        // in registers until its place in the sorted order is known.
+        int key[MAXM];
+        Real stash[MAXM][RL::len()];
+#pragma unroll
+        for ( int mi = 0; mi < MAXM; ++mi )
         {
-            const int64_t p =
-                run_particle( ta, run_start, run_pre, nruns, bstart + i );
+            const int i = (int)threadIdx.x + mi * BLOCK;
+            key[mi] = -1;
+            if ( i >= nbatch )
+                continue;
+            const int64_t p = run_particle( ta, run_start, run_pre, nruns, pbeg + i );
             double px[3];
             load_position( a.points, p, px );
             int bs[3];
-            int k = -1;
-            if ( !stencil_base<ORDER>( a.mesh, px, bs ) )
+            bool ok = true;
+#pragma unroll
+            for ( int d = 0; d < 3; ++d )
+                ok = spline_frac<ORDER, Real>(
+                         logical_coord<ORDER>( px[d], a.mesh.low[d], a.mesh.rdx[d] ),
+                         a.mesh.num_node[d], bs[d], stash[mi][d] ) &&
+                     ok;
+            if ( !ok )
             {
                 atomicAdd( a.skipped, 1ull );
-            }
-            else
-            {
-                int l[3];
-                bool inside = true;
-#pragma unroll
-                for ( int d = 0; d < 3; ++d )
-                {
-                    l[d] = bs[d] - blo[d];
-                    inside = inside && l[d] >= 0 && l[d] < NB[d];
-                }
-                if ( inside )
-                {
-                    const int cell = ( l[0] * NB[1] + l[1] ) * NB[2] + l[2];
-                    const int slot = atomicAdd( cnt + cell, 1 );
-                    k = ( cell << 16 ) | slot;
-                }
-                else
-                {
-                    // Not sorted into this brick: add straight to the grid.
-                    Spline3D<ORDER, Real, false> s;
-                    eval_spline( a.mesh, px, s );
-                    global_scatter_particle<Real, SHARED, FORMS...>( a, p, s );
-                }
-            }
-            key[i] = k;
-        }
-        __syncthreads();
-        // B: base-cell offsets; cnt[nbt] = number of sorted particles.
-        const int nsorted = block_exclusive_scan( cnt, nbt, wsum );
-        if ( threadIdx.x == 0 )
-            cnt[nbt] = nsorted;
-        // C: particles (index relative to the binned range) listed by base
-        // cell.
-        for ( int i = threadIdx.x; i < nbatch; i += BLOCK )
-        {
-            const int k = key[i];
-            if ( k >= 0 )
-                list[cnt[k >> 16] + ( k & 0xffff )] = (int)(
-                    run_particle( ta, run_start, run_pre, nruns, bstart + i ) -
-                    ta.bin_begin );
-        }
-        __syncthreads();
-        // D: one lane per (base cell, x-slab): accumulate the slab's K^2
-        // nodes over the cell's particles in registers, flush once.
-        for ( int task = threadIdx.x; task < nbt * K; task += BLOCK )
-        {
-            const int cell = task / K;
-            const int sa = task - cell * K;
-            const int q0 = cnt[cell], q1 = cnt[cell + 1];
-            if ( q0 == q1 )
                 continue;
-            Real acc[K * K][DT];
+            }
+            int l[3];
+            bool inside = true;
 #pragma unroll
-            for ( int n = 0; n < K * K; ++n )
-#pragma unroll
-                for ( int d = 0; d < DT; ++d )
-                    acc[n][d] = Real( 0 );
-            for ( int q = q0; q < q1; ++q )
+            for ( int d = 0; d < 3; ++d )
             {
-                const int64_t p = ta.bin_begin + list[q];
-                double px[3];
-                load_position( a.points, p, px );
-                Spline1D<ORDER, Real> sx, sy, sz;
-                eval_spline_1d<ORDER, Real, false>(
-                    logical_coord<ORDER>( px[0], a.mesh.low[0], a.mesh.rdx[0] ),
-                    (Real)a.mesh.rdx[0], a.mesh.num_node[0], sx );
-                eval_spline_1d<ORDER, Real, false>(
-                    logical_coord<ORDER>( px[1], a.mesh.low[1], a.mesh.rdx[1] ),
-                    (Real)a.mesh.rdx[1], a.mesh.num_node[1], sy );
-                eval_spline_1d<ORDER, Real, false>(
-                    logical_coord<ORDER>( px[2], a.mesh.low[2], a.mesh.rdx[2] ),
-                    (Real)a.mesh.rdx[2], a.mesh.num_node[2], sz );
-                const Real wx = pick( sx.w, sa );
-                const Real gx = pick( sx.g, sa );
-                Real Pm[NT][9];
+                l[d] = bs[d] - blo[d];
+                inside = inside && l[d] >= 0 && l[d] < NB[d];
+            }
+            if ( inside )
+            {
+                const int cell = ( l[0] * NB[1] + l[1] ) * NB[2] + l[2];
+                key[mi] = ( cell << 16 ) | atomicAdd( cnt + cell, 1 );
                 static_for<NT>( [&]( auto I ) {
                     constexpr int t = I;
                     constexpr int NP = np_of( FL::v[t] );
                     const Real m = (Real)a.t[t].mult;
-                    const int64_t b = a.t[t].pin.base( p );
+                    const uint32_t b = a.t[t].pin.base( p );
 #pragma unroll
                     for ( int c = 0; c < NP; ++c )
-                        Pm[t][c] = *a.t[t].pin.template ptr<const Real>( b, c ) * m;
+                        stash[mi][RL::attr( t ) + c] =
+                            *a.t[t].pin.template ptr<const Real>( b, c ) * m;
                 } );
-#pragma unroll
-                for ( int j = 0; j < K; ++j )
-#pragma unroll
-                    for ( int k = 0; k < K; ++k )
-                    {
-                        NodeW<Real, false> n;
-                        const Real wy = sy.w[j], wz = sz.w[k];
-                        const Real wyz = wy * wz;
-                        n.W = wx * wyz;
-                        n.dW[0] = gx * wyz;
-                        n.dW[1] = wx * sy.g[j] * wz;
-                        n.dW[2] = wx * wy * sz.g[k];
-                        static_for<NT>( [&]( auto I ) {
-                            constexpr int t = I;
-                            constexpr int FORM = FL::v[t];
-                            constexpr int NG = ng_of( FORM );
-                            constexpr int O = SD::offset( t, SHARED );
-                            Real G[NG];
-                            form_scatter<FORM>( Pm[t], n, G );
-#pragma unroll
-                            for ( int d = 0; d < NG; ++d )
-                                acc[j * K + k][O + d] += G[d];
-                        } );
-                    }
             }
-            const int lx = cell / ( NB[1] * NB[2] );
-            const int ly = ( cell / NB[2] ) % NB[1];
-            const int lz = cell % NB[2];
-#pragma unroll
-            for ( int j = 0; j < K; ++j )
-#pragma unroll
-                for ( int k = 0; k < K; ++k )
-                {
-                    const int node = ( ( lx + sa ) * W[1] + ly + j ) * W[2] + lz + k;
-#pragma unroll
-                    for ( int d = 0; d < DT; ++d )
-                        atomicAdd( win + node * DT + d, (double)acc[j * K + k][d] );
-                }
+            else
+            {
+                // Not sorted into this brick: add straight to the grid.
+                Spline3D<ORDER, Real, false> sp;
+                eval_spline( a.mesh, px, sp );
+                global_scatter_particle<Real, SHARED, FORMS...>( a, p, sp );
+            }
         }
         __syncthreads();
-    }
-
-    // Flush the window: one global atomic per non-zero node value, walking the
-    // window with the grid's unit-stride dimension fastest.
-    static_for<( SHARED ? 1 : NT )>( [&]( auto I ) {
-        constexpr int t = I;
-        constexpr int NG = SHARED ? 3 : ng_of( FL::v[t] );
-        constexpr int O = SD::offset( t, SHARED );
-        const GFieldDev& g = a.t[t].grid;
-        for ( int f = threadIdx.x; f < wnodes * NG; f += BLOCK )
+        // B: base-cell offsets (cnt[nbt] = sorted particles), then the task
+        // table: every base cell's records cut into runs of at most L.
+        const int nsorted = block_exclusive_scan( cnt, nbt, wsum );
+        if ( threadIdx.x == 0 )
+            cnt[nbt] = nsorted;
+        __syncthreads();
+        for ( int b = threadIdx.x; b < nbt; b += BLOCK )
+            sub[b] = ( cnt[b + 1] - cnt[b] + L - 1 ) / L;
+        __syncthreads();
+        const int ntask = block_exclusive_scan( sub, nbt, wsum );
+        for ( int b = threadIdx.x; b < nbt; b += BLOCK )
         {
-            int q[3], d;
-            window_coords( ta, f, NG, W, q, d );
-            const Real v =
-                (Real)win[( ( q[0] * W[1] + q[1] ) * W[2] + q[2] ) * DT + O + d];
-            if ( v == Real( 0 ) )
-                continue;
-            const int gi = blo[0] + q[0], gj = blo[1] + q[1], gk = blo[2] + q[2];
-            if ( gi < 0 || gj < 0 || gk < 0 || gi >= a.mesh.num_node[0] ||
-                 gj >= a.mesh.num_node[1] || gk >= a.mesh.num_node[2] )
-                continue;
-            atomic_add( gptr<Real>( g, gi * g.s[0] +
-                            gj * g.s[1] + gk * g.s[2] + d * g.s[3] ),
-                        v );
+            const int c0 = cnt[b], c = cnt[b + 1] - c0;
+            int q = sub[b];
+            for ( int r = 0; r < c; r += L )
+                task[q++] = pack_task( b, c0 + r, min( L, c - r ) );
         }
-    } );
+        // C: records in base-cell order.
+#pragma unroll
+        for ( int mi = 0; mi < MAXM; ++mi )
+        {
+            if ( key[mi] < 0 )
+                continue;
+            Real* rp = rec + (size_t)RS * ( cnt[key[mi] >> 16] + ( key[mi] & 0xffff ) );
+#pragma unroll
+            for ( int c = 0; c < RL::len(); ++c )
+                rp[c] = stash[mi][c];
+        }
+        __syncthreads();
+        // D: one lane per (run, group[, x-slab]): the run's particles
+        // accumulated over the stencil (or one x-slab of it) in registers,
+        // then added once into the window.  Each group's tasks start on a
+        // wave boundary, so a wave runs one group's code.
+        int seg[NGROUP + 1];
+        seg[0] = 0;
+        static_for<NGROUP>( [&]( auto GI ) {
+            constexpr int grp = GI;
+            constexpr int DG = SHARED ? 3 : ng_of( FL::v[grp] );
+            constexpr int S = ( DG > 1 ) ? K : 1;
+            seg[grp + 1] = seg[grp] + ( ntask * S + 63 ) / 64 * 64;
+        } );
+        for ( int tt = threadIdx.x; tt < seg[NGROUP]; tt += BLOCK )
+        {
+            static_for<NGROUP>( [&]( auto GI ) {
+                constexpr int grp = GI;
+                constexpr int DG = SHARED ? 3 : ng_of( FL::v[grp] );
+                constexpr int O = SHARED ? 0 : SD::offset( grp, false );
+                constexpr bool SLABS = DG > 1;
+                constexpr int S = SLABS ? K : 1;
+                constexpr int NI = SLABS ? 1 : K;
+                const int local = tt - seg[grp];
+                if ( local < 0 || local >= ntask * S )
+                    return;
+                const int q = SLABS ? local / K : local;
+                const int sa = SLABS ? local - q * K : 0;
+                const unsigned e = task[q];
+                const int b = (int)( e >> 20 );
+                const int r0 = (int)( ( e >> 5 ) & 0x7fff );
+                const int r1 = r0 + (int)( e & 31 );
+                Real acc[NI * K * K][DG];
+#pragma unroll
+                for ( int n = 0; n < NI * K * K; ++n )
+#pragma unroll
+                    for ( int d = 0; d < DG; ++d )
+                        acc[n][d] = Real( 0 );
+                for ( int r = r0; r < r1; ++r )
+                {
+                    const Real* rp = rec + (size_t)RS * r;
+                    Spline1D<ORDER, Real> sp[3];
+#pragma unroll
+                    for ( int d = 0; d < 3; ++d )
+                        spline_weights<ORDER, Real, false>( rp[d], rdx[d], sp[d] );
+                    Real wx[NI], gx[NI];
+                    if constexpr ( SLABS )
+                    {
+                        wx[0] = sp[0].w[0];
+                        gx[0] = sp[0].g[0];
+#pragma unroll
+                        for ( int ii = 1; ii < K; ++ii )
+                        {
+                            wx[0] = ( sa == ii ) ? sp[0].w[ii] : wx[0];
+                            gx[0] = ( sa == ii ) ? sp[0].g[ii] : gx[0];
+                        }
+                    }
+                    else
+                    {
+#pragma unroll
+                        for ( int i = 0; i < NI; ++i )
+                        {
+                            wx[i] = sp[0].w[i];
+                            gx[i] = sp[0].g[i];
+                        }
+                    }
+                    Real Pm[NT][9];
+                    static_for<NT>( [&]( auto I ) {
+                        constexpr int t = I;
+                        if constexpr ( SHARED || t == grp )
+                        {
+#pragma unroll
+                            for ( int c = 0; c < np_of( FL::v[t] ); ++c )
+                                Pm[t][c] = rp[RL::attr( t ) + c];
+                        }
+                    } );
+#pragma unroll
+                    for ( int i = 0; i < NI; ++i )
+#pragma unroll
+                        for ( int j = 0; j < K; ++j )
+#pragma unroll
+                            for ( int k = 0; k < K; ++k )
+                            {
+                                NodeW<Real, false> n;
+                                const Real wy = sp[1].w[j], wz = sp[2].w[k];
+                                const Real wyz = wy * wz;
+                                n.W = wx[i] * wyz;
+                                n.dW[0] = gx[i] * wyz;
+                                n.dW[1] = wx[i] * sp[1].g[j] * wz;
+                                n.dW[2] = wx[i] * wy * sp[2].g[k];
+                                static_for<NT>( [&]( auto I ) {
+                                    constexpr int t = I;
+                                    if constexpr ( SHARED || t == grp )
+                                    {
+                                        constexpr int FORM = FL::v[t];
+                                        Real G[ng_of( FORM )];
+                                        form_scatter<FORM>( Pm[t], n, G );
+#pragma unroll
+                                        for ( int d = 0; d < DG; ++d )
+                                            acc[( i * K + j ) * K + k][d] += G[d];
+                                    }
+                                } );
+                            }
+                }
+                const int lx = b / ( NB[1] * NB[2] );
+                const int ly = ( b / NB[2] ) % NB[1];
+                const int lz = b % NB[2];
+#pragma unroll
+                for ( int i = 0; i < NI; ++i )
+#pragma unroll
+                    for ( int j = 0; j < K; ++j )
+#pragma unroll
+                        for ( int k = 0; k < K; ++k )
+                        {
+                            const int node =
+                                ( ( lx + sa + i ) * W[1] + ly + j ) * W[2] + lz + k;
+#pragma unroll
+                            for ( int d = 0; d < DG; ++d )
+                                atomicAdd( win + node * DT + O + d,
+                                           (double)acc[( i * K + j ) * K + k][d] );
+                        }
+            } );
+        }
+        __syncthreads();
+
+        // E: flush the window, one global atomic per non-zero node value,
+        // walking the window with the grid's unit-stride dimension fastest.
+        static_for<( SHARED ? 1 : NT )>( [&]( auto I ) {
+            constexpr int t = I;
+            constexpr int NG = SHARED ? 3 : ng_of( FL::v[t] );
+            constexpr int O = SD::offset( t, SHARED );
+            const GFieldDev& g = a.t[t].grid;
+            for ( int f = threadIdx.x; f < wnodes * NG; f += BLOCK )
+            {
+                int q[3], d;
+                window_coords( ta, f, NG, W, q, d );
+                const Real v =
+                    (Real)win[( ( q[0] * W[1] + q[1] ) * W[2] + q[2] ) * DT + O + d];
+                if ( v == Real( 0 ) )
+                    continue;
+                const int gi = blo[0] + q[0], gj = blo[1] + q[1], gk = blo[2] + q[2];
+                if ( gi < 0 || gj < 0 || gk < 0 || gi >= a.mesh.num_node[0] ||
+                     gj >= a.mesh.num_node[1] || gk >= a.mesh.num_node[2] )
+                    continue;
+                atomic_add( gptr<Real>( g, gi * g.s[0] + gj * g.s[1] + gk * g.s[2] +
+                                               d * g.s[3] ),
+                            v );
+            }
+        } );
+        // The next part reuses the window and the tables.
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -632,8 +808,7 @@ hipError_t launch_tile_kernel( KernelPtr k, const Args& a, const TileArgs& t,
         if ( e != hipSuccess )
             return e;
     }
-    hipLaunchKernelGGL( k, dim3( (unsigned)( 8 * t.tiles_per_xcd ) ),
-                        dim3( BLOCK ), lds, s, a, t );
+    hipLaunchKernelGGL( k, dim3( (unsigned)t.grid ), dim3( BLOCK ), lds, s, a, t );
     return hipGetLastError();
 }
 
@@ -641,6 +816,7 @@ template <class Real>
 hipError_t gather_tiled_typed( Launcher& L, int plan, const GatherArgs& a,
                                const TileArgs& t, size_t lds, hipStream_t s )
 {
+    const int mode = !a.has_xbar ? GATHER_OUT : ( a.t[0].pout.data ? GATHER_OUT_XBAR : GATHER_XBAR );
     const char* name =
         a.has_xbar ? ( plan == PLAN_MPM_P2G ? "p2g_adjoint_mpm_tiled" : "gather_xbar_tiled" )
                    : ( plan == PLAN_MPM_G2P ? "g2p_mpm_tiled" : "g2p_tiled" );
@@ -652,10 +828,14 @@ hipError_t gather_tiled_typed( Launcher& L, int plan, const GatherArgs& a,
         {
 #define CA_CASE( F )                                                           \
     case F:                                                                    \
-        e = a.has_xbar ? launch_tile_kernel( gather_tiled_kernel<Real, true, false, F>, \
-                                             a, t, lds, s )                    \
-                       : launch_tile_kernel( gather_tiled_kernel<Real, false, false, F>, \
-                                             a, t, lds, s );                   \
+        e = mode == GATHER_OUT                                                 \
+                ? launch_tile_kernel( gather_tiled_kernel<Real, GATHER_OUT, false, F>, a, \
+                                      t, lds, s )                              \
+            : mode == GATHER_OUT_XBAR                                          \
+                ? launch_tile_kernel( gather_tiled_kernel<Real, GATHER_OUT_XBAR, false, F>, \
+                                      a, t, lds, s )                           \
+                : launch_tile_kernel( gather_tiled_kernel<Real, GATHER_XBAR, false, F>, a, \
+                                      t, lds, s );                             \
         break;
             CA_CASE( FORM_SV )
             CA_CASE( FORM_VV )
@@ -668,16 +848,16 @@ hipError_t gather_tiled_typed( Launcher& L, int plan, const GatherArgs& a,
             e = hipErrorInvalidValue;
         }
     }
-    else if ( plan == PLAN_MPM_G2P && !a.has_xbar )
-        e = launch_tile_kernel( gather_tiled_kernel<Real, false, true, FORM_VV, FORM_TDT>,
+    else if ( plan == PLAN_MPM_G2P && mode == GATHER_OUT )
+        e = launch_tile_kernel( gather_tiled_kernel<Real, GATHER_OUT, true, FORM_VV, FORM_TDT>,
                                 a, t, lds, s );
-    else if ( plan == PLAN_MPM_G2P && a.has_xbar )
-        e = launch_tile_kernel( gather_tiled_kernel<Real, true, true, FORM_VV, FORM_TDT>,
+    else if ( plan == PLAN_MPM_G2P && mode == GATHER_XBAR )
+        e = launch_tile_kernel( gather_tiled_kernel<Real, GATHER_XBAR, true, FORM_VV, FORM_TDT>,
                                 a, t, lds, s );
-    else if ( plan == PLAN_MPM_P2G && a.has_xbar )
+    else if ( plan == PLAN_MPM_P2G && mode == GATHER_OUT_XBAR )
         e = launch_tile_kernel(
-            gather_tiled_kernel<Real, true, false, FORM_SV, FORM_VV, FORM_TD>, a, t,
-            lds, s );
+            gather_tiled_kernel<Real, GATHER_OUT_XBAR, false, FORM_SV, FORM_VV, FORM_TD>, a,
+            t, lds, s );
     else
         e = hipErrorInvalidValue;
     L.after( name, s );

@@ -93,7 +93,10 @@ typedef struct
  * which is Cabana_Slice.hpp:119-190.  An AoSoA slice has vector_length = VL
  * and soa_stride = slice.stride(0); an (n, ncomp) row-major array has
  * vector_length = 1, soa_stride = ncomp; a column-major one has
- * vector_length = soa_stride = n. */
+ * vector_length = soa_stride = n.  vector_length must be a power of two (as
+ * for every Cabana AoSoA) or at least num_particle, num_particle below 2^31
+ * and every element index below 2^32 (CABANA_AMD_ERROR_UNSUPPORTED
+ * otherwise). */
 typedef struct
 {
     void* data;
@@ -117,11 +120,13 @@ typedef struct
 } cabana_amd_grid_field;
 
 /* Cell binning of the particles (the output of
- * cabana_amd_linked_cell_build followed by cabana_amd_permute).  When passed
- * to the interpolation calls, particles [begin, end) must be stored in
- * cardinal cell order (i slowest, k fastest) over the mesh's own cells, which
- * lets the kernels walk cell-sorted tiles.  Particles outside [begin, end)
- * are still processed.  NULL means "no ordering is known". */
+ * cabana_amd_linked_cell_build followed by cabana_amd_permute), over the
+ * mesh's own cells.  The interpolation calls then walk particles [begin, end)
+ * as cell-sorted bricks; the binning only affects speed: particles outside
+ * [begin, end), and particles that moved after the binning was built, are
+ * interpolated exactly like the rest (through global memory instead of the
+ * brick's LDS window).  counts/offsets must still partition [begin, end).
+ * NULL means "no ordering is known". */
 typedef struct
 {
     const int32_t* counts;  /* per cell */

@@ -226,10 +226,11 @@ def mpm_fields(c, dtype):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("order", [1, 2, 3])
-@pytest.mark.parametrize("sorted_bins", [False, True])
+@pytest.mark.parametrize("sorted_bins", [False, True, "stale"])
 def test_fused_mpm_transfer(ca, oracle, order, dtype, sorted_bins):
     """P2G (mass, momentum, stress divergence) and G2P (velocity, velocity
-    gradient) as fused passes, with and without cell-sorted tiles."""
+    gradient) as fused passes, with and without cell-sorted tiles, and with a
+    stale binning (particles moved by up to 1.5 cells after the sort)."""
     c = Case(ca, oracle, num_cell=(12, 10, 9), n=6000, seed=9 + order)
     m, q, sig = mpm_fields(c, dtype)
     pts = c.pts
@@ -245,6 +246,11 @@ def test_fused_mpm_transfer(ca, oracle, order, dtype, sorted_bins):
         pts_np = pts_np[perm]
         m, q, sig = m[perm], q[perm], sig[perm]
         bins = lcl
+        if sorted_bins == "stale":
+            lo_a, hi_a = np.array(lo), np.array(hi)
+            step = c.rng.uniform(-1.5, 1.5, size=pts_np.shape) * gm.cellSize(0)
+            pts_np = lo_a + np.mod(pts_np + step - lo_a, hi_a - lo_a)
+            pts = torch.tensor(pts_np, device="cuda")
     tm = torch.tensor(m[:, 0], dtype=dtype, device="cuda")
     tq = torch.tensor(q, dtype=dtype, device="cuda")
     ts = torch.tensor(sig.reshape(-1, 3, 3), dtype=dtype, device="cuda")
