@@ -684,54 +684,74 @@ struct GatherNeeds
     }
 };
 
-// One particle of a gather pass: for each grid slot, its jets, then for
-// each term reading that slot
-//   pout += mult * dB/dP      (MODE has outputs)
-//   xbar += mult * dB/dx      (MODE has the position cotangent).
+// Global reads of one particle in a gather pass: the accumulated outputs,
+// the attributes the position adjoint needs, the position cotangent.
+template <class Real, int MODE, int... FORMS>
+struct GatherIn
+{
+    static constexpr int NT = sizeof...( FORMS );
+    Real old[NT][9];
+    Real P[NT][9];
+    double xb[3];
+};
+
+template <class Real, int MODE, int... FORMS>
+__device__ __forceinline__ void gather_load( const GatherArgs& a, int64_t p,
+                                             GatherIn<Real, MODE, FORMS...>& in )
+{
+    using GN = GatherNeeds<FORMS...>;
+    constexpr int NT = GN::N;
+    static_for<NT>( [&]( auto I ) {
+        constexpr int t = I;
+        constexpr int NP = FormTraits<GN::f[t]>::NP;
+        if constexpr ( mode_out( MODE ) )
+        {
+            const PFieldDev& o = a.t[t].pout;
+            const uint32_t b = o.base( p );
+#pragma unroll
+            for ( int c = 0; c < NP; ++c )
+                in.old[t][c] = *o.template ptr<const Real>( b, c );
+        }
+        if constexpr ( mode_xbar( MODE ) )
+        {
+            const uint32_t b = a.t[t].px.base( p );
+#pragma unroll
+            for ( int c = 0; c < NP; ++c )
+                in.P[t][c] = *a.t[t].px.template ptr<const Real>( b, c );
+        }
+    } );
+    if constexpr ( mode_xbar( MODE ) )
+    {
+        const uint32_t b = a.xbar.base( p );
+#pragma unroll
+        for ( int e = 0; e < 3; ++e )
+            in.xb[e] = *a.xbar.ptr<const double>( b, e );
+    }
+}
+
+// One particle of a gather pass, from its global reads `in`: for each grid
+// slot, its jets, then for each term reading that slot
+//   pout = old + mult * dB/dP      (MODE has outputs)
+//   xbar = old + mult * dB/dx      (MODE has the position cotangent).
 // Slots are finished one after the other, so only one slot's jets are live.
 // Load(slot_constant, i, j, k, f) fills the slot's NG values of stencil node
 // (i, j, k).
 template <int ORDER, class Real, int MODE, bool SHARED, int... FORMS, class Load>
-__device__ __forceinline__ void gather_particle(
-    const GatherArgs& a, int64_t p,
-    const Spline3D<ORDER, Real, mode_xbar( MODE )>& s, Load&& load )
+__device__ __forceinline__ void gather_compute(
+    const GatherArgs& a, int64_t p, const Spline3D<ORDER, Real, mode_xbar( MODE )>& s,
+    Load&& load, const GatherIn<Real, MODE, FORMS...>& in )
 {
     using GN = GatherNeeds<FORMS...>;
     constexpr int NT = GN::N;
     constexpr int NSLOT = SHARED ? 1 : NT;
     constexpr bool XBAR = mode_xbar( MODE );
     constexpr bool OUT = mode_out( MODE );
-    // Every global read of the particle (accumulated outputs, attributes of
-    // the position adjoint, the position cotangent) is issued before the
-    // stencil work, so its latency overlaps the jets.
-    Real old[NT][9];
-    Real P[NT][9];
-    static_for<NT>( [&]( auto I ) {
-        constexpr int t = I;
-        constexpr int NP = FormTraits<GN::f[t]>::NP;
-        if constexpr ( OUT )
-        {
-            const PFieldDev& o = a.t[t].pout;
-            const uint32_t b = o.base( p );
-#pragma unroll
-            for ( int c = 0; c < NP; ++c )
-                old[t][c] = *o.template ptr<const Real>( b, c );
-        }
-        if constexpr ( XBAR )
-        {
-            const uint32_t b = a.t[t].px.base( p );
-#pragma unroll
-            for ( int c = 0; c < NP; ++c )
-                P[t][c] = *a.t[t].px.template ptr<const Real>( b, c );
-        }
-    } );
     double xb[3] = { 0.0, 0.0, 0.0 };
     if constexpr ( XBAR )
     {
-        const uint32_t b = a.xbar.base( p );
 #pragma unroll
         for ( int e = 0; e < 3; ++e )
-            xb[e] = *a.xbar.ptr<const double>( b, e );
+            xb[e] = in.xb[e];
     }
     static_for<NSLOT>( [&]( auto S ) {
         constexpr int sl = S;
@@ -755,12 +775,12 @@ __device__ __forceinline__ void gather_particle(
                     const uint32_t b = o.base( p );
 #pragma unroll
                     for ( int c = 0; c < NP; ++c )
-                        *o.template ptr<Real>( b, c ) = old[t][c] + m * out[c];
+                        *o.template ptr<Real>( b, c ) = in.old[t][c] + m * out[c];
                 }
                 if constexpr ( XBAR )
                 {
                     Real xt[3] = { Real( 0 ), Real( 0 ), Real( 0 ) };
-                    jet_dx<F>( P[t], J, xt );
+                    jet_dx<F>( in.P[t], J, xt );
 #pragma unroll
                     for ( int e = 0; e < 3; ++e )
                         xb[e] += (double)( m * xt[e] );
@@ -777,6 +797,18 @@ __device__ __forceinline__ void gather_particle(
         for ( int e = 0; e < 3; ++e )
             *a.xbar.ptr<double>( b, e ) = xb[e];
     }
+}
+
+// One particle of a gather pass: its global reads (issued before the stencil
+// work so their latency overlaps the jets), then the jets and the updates.
+template <int ORDER, class Real, int MODE, bool SHARED, int... FORMS, class Load>
+__device__ __forceinline__ void gather_particle(
+    const GatherArgs& a, int64_t p, const Spline3D<ORDER, Real, mode_xbar( MODE )>& s,
+    Load&& load )
+{
+    GatherIn<Real, MODE, FORMS...> in;
+    gather_load<Real, MODE, FORMS...>( a, p, in );
+    gather_compute<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, load, in );
 }
 
 // Loader of stencil nodes straight from the grid arrays.

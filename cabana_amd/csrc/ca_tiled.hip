@@ -274,29 +274,35 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
         const int pend = min( total, pbeg + ta.cap );
 
         // Lanes stride through the part; the cell run of the next particle
-        // is found by walking forward, and its position is loaded one
-        // iteration ahead.
+        // is found by walking forward, and its position and other global
+        // reads are issued one iteration ahead, so their latency overlaps
+        // the current particle's stencil work.
+        using In = GatherIn<Real, MODE, FORMS...>;
         int v = pbeg + (int)threadIdx.x;
         int r = 0;
         int64_t p = 0;
         double px[3] = { 0.0, 0.0, 0.0 };
+        In in;
         if ( v < pend )
         {
             r = find_run( run_pre, nruns, v );
             p = ta.bin_begin + run_start[r] + ( v - run_pre[r] );
             load_position( a.points, p, px );
+            gather_load<Real, MODE, FORMS...>( a, p, in );
         }
         while ( v < pend )
         {
             const int vn = v + BLOCK;
             int64_t pn = 0;
             double pxn[3] = { 0.0, 0.0, 0.0 };
+            In inn;
             if ( vn < pend )
             {
                 while ( run_pre[r + 1] <= vn )
                     ++r;
                 pn = ta.bin_begin + run_start[r] + ( vn - run_pre[r] );
                 load_position( a.points, pn, pxn );
+                gather_load<Real, MODE, FORMS...>( a, pn, inn );
             }
             Spline3D<ORDER, Real, mode_xbar( MODE )> s;
             if ( !eval_spline( a.mesh, px, s ) )
@@ -338,13 +344,13 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
                             f[2] = q[2];
                         }
                     };
-                    gather_particle<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, ld );
+                    gather_compute<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, ld, in );
                 }
                 else
                 {
                     GlobalGatherLoad<Real, SHARED, FORMS...> ld{
                         a, { s.d[0].base, s.d[1].base, s.d[2].base } };
-                    gather_particle<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, ld );
+                    gather_compute<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, ld, in );
                 }
             }
             v = vn;
@@ -352,6 +358,7 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
 #pragma unroll
             for ( int d = 0; d < 3; ++d )
                 px[d] = pxn[d];
+            in = inn;
         }
         // The next part restages the windows.
         __syncthreads();
@@ -464,6 +471,63 @@ struct SlotDofs
         return shared ? 3 : o;
     }
 };
+
+__device__ __forceinline__ float fma_r( float a, float b, float c ) { return fmaf( a, b, c ); }
+__device__ __forceinline__ double fma_r( double a, double b, double c ) { return fma( a, b, c ); }
+
+// Scatter contribution of one particle to the nodes of one x-slab (fixed
+// stencil index i, 1-D weight wx and gradient gx): with A = wy*wz,
+// B = gy*wz, C = wy*gz over the (j, k) nodes of the slab, the node weight is
+// W = wx*A and its gradient (gx*A, wx*B, wx*C), so every form's node value is
+// G[d] = A*u[d] + B*v[d] + C*w[d] for slab coefficients u, v, w formed once
+// per particle and slab (dB/dG of the forms in ca_device.hpp, regrouped).
+template <int FORM, class Real>
+__device__ __forceinline__ void slab_coeffs( const Real* P, Real wx, Real gx, Real* u,
+                                             Real* v, Real* w )
+{
+    if constexpr ( FORM == FORM_SV )
+    {
+        u[0] += wx * P[0];
+    }
+    else if constexpr ( FORM == FORM_VV )
+    {
+#pragma unroll
+        for ( int d = 0; d < 3; ++d )
+            u[d] += wx * P[d];
+    }
+    else if constexpr ( FORM == FORM_SG )
+    {
+        u[0] += gx * P[0];
+        v[1] += wx * P[0];
+        w[2] += wx * P[0];
+    }
+    else if constexpr ( FORM == FORM_VD )
+    {
+        u[0] += gx * P[0];
+        v[0] += wx * P[1];
+        w[0] += wx * P[2];
+    }
+    else if constexpr ( FORM == FORM_TD )
+    {
+#pragma unroll
+        for ( int a = 0; a < 3; ++a )
+        {
+            u[a] += gx * P[3 * a];
+            v[a] += wx * P[3 * a + 1];
+            w[a] += wx * P[3 * a + 2];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for ( int b = 0; b < 3; ++b )
+        {
+            u[b] += gx * P[b];
+            v[b] += wx * P[3 + b];
+            w[b] += wx * P[6 + b];
+        }
+    }
+}
 
 // Record layout of a scatter pass: 3 fractional offsets, then the
 // attributes of every term in order.
@@ -661,6 +725,13 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
                 constexpr bool SLABS = DG > 1;
                 constexpr int S = SLABS ? K : 1;
                 constexpr int NI = SLABS ? 1 : K;
+                // Gradient-type forms in the group need the B, C products.
+                constexpr bool GRAD = [] {
+                    for ( int t = 0; t < NT; ++t )
+                        if ( ( SHARED || t == grp ) && !form_value_type( FL::v[t] ) )
+                            return true;
+                    return false;
+                }();
                 const int local = tt - seg[grp];
                 if ( local < 0 || local >= ntask * S )
                     return;
@@ -714,33 +785,49 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
                                 Pm[t][c] = rp[RL::attr( t ) + c];
                         }
                     } );
+                    // (j, k) products of the slab: A always, B and C for
+                    // gradient-type forms.
+                    Real A[K][K], B[K][K], C[K][K];
+#pragma unroll
+                    for ( int j = 0; j < K; ++j )
+#pragma unroll
+                        for ( int k = 0; k < K; ++k )
+                        {
+                            A[j][k] = sp[1].w[j] * sp[2].w[k];
+                            if constexpr ( GRAD )
+                            {
+                                B[j][k] = sp[1].g[j] * sp[2].w[k];
+                                C[j][k] = sp[1].w[j] * sp[2].g[k];
+                            }
+                        }
 #pragma unroll
                     for ( int i = 0; i < NI; ++i )
+                    {
+                        Real u[DG], v[DG], w[DG];
+#pragma unroll
+                        for ( int d = 0; d < DG; ++d )
+                            u[d] = v[d] = w[d] = Real( 0 );
+                        static_for<NT>( [&]( auto I ) {
+                            constexpr int t = I;
+                            if constexpr ( SHARED || t == grp )
+                                slab_coeffs<FL::v[t]>( Pm[t], wx[i], gx[i], u, v, w );
+                        } );
 #pragma unroll
                         for ( int j = 0; j < K; ++j )
 #pragma unroll
                             for ( int k = 0; k < K; ++k )
-                            {
-                                NodeW<Real, false> n;
-                                const Real wy = sp[1].w[j], wz = sp[2].w[k];
-                                const Real wyz = wy * wz;
-                                n.W = wx[i] * wyz;
-                                n.dW[0] = gx[i] * wyz;
-                                n.dW[1] = wx[i] * sp[1].g[j] * wz;
-                                n.dW[2] = wx[i] * wy * sp[2].g[k];
-                                static_for<NT>( [&]( auto I ) {
-                                    constexpr int t = I;
-                                    if constexpr ( SHARED || t == grp )
-                                    {
-                                        constexpr int FORM = FL::v[t];
-                                        Real G[ng_of( FORM )];
-                                        form_scatter<FORM>( Pm[t], n, G );
 #pragma unroll
-                                        for ( int d = 0; d < DG; ++d )
-                                            acc[( i * K + j ) * K + k][d] += G[d];
+                                for ( int d = 0; d < DG; ++d )
+                                {
+                                    Real& c = acc[( i * K + j ) * K + k][d];
+                                    c = fma_r( A[j][k], u[d], c );
+                                    if constexpr ( GRAD )
+                                    {
+                                        c = fma_r( B[j][k], v[d], c );
+                                        c = fma_r( C[j][k], w[d], c );
                                     }
-                                } );
-                            }
+                                }
+                    }
                 }
                 const int lx = b / ( NB[1] * NB[2] );
                 const int ly = ( b / NB[2] ) % NB[1];
