@@ -693,6 +693,10 @@ int slot_dofs( int plan, const int* forms, int nterms, int* ng )
     return nterms;
 }
 
+// True when a particle field is absent or stores each particle's components
+// contiguously (vector_length 1): the tiled kernels then move them as vectors.
+bool pf_contig( const PFieldDev& f ) { return !f.data || f.vl == 1; }
+
 // Scatter pass over all particles: cell-register tiles over
 // [bins.begin, bins.end) when the binning matches the mesh, atomics elsewhere.
 cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
@@ -704,6 +708,9 @@ cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
     for ( int i = 0; i < a.nterms; ++i )
         forms[i] = a.t[i].form;
     const int nslot = slot_dofs( plan, forms, a.nterms, ng );
+    a.contig = pf_contig( a.points ) && pf_contig( a.xbar );
+    for ( int i = 0; i < a.nterms; ++i )
+        a.contig = a.contig && pf_contig( a.t[i].pin );
     int rec_len = 3;
     for ( int i = 0; i < a.nterms; ++i )
         rec_len += form_pcomp( forms[i] );
@@ -754,6 +761,9 @@ cabana_amd_status gather_pass( cabana_amd_context* ctx, int order, bool fp64,
     for ( int i = 0; i < a.nterms; ++i )
         forms[i] = a.t[i].form;
     const int nslot = slot_dofs( plan, forms, a.nterms, ng );
+    a.contig = pf_contig( a.points ) && pf_contig( a.xbar );
+    for ( int i = 0; i < a.nterms; ++i )
+        a.contig = a.contig && pf_contig( a.t[i].pout ) && pf_contig( a.t[i].px );
     TileArgs t;
     size_t lds = 0;
     const bool tiled =

@@ -695,7 +695,7 @@ struct GatherIn
     double xb[3];
 };
 
-template <class Real, int MODE, int... FORMS>
+template <class Real, int MODE, bool CONTIG, int... FORMS>
 __device__ __forceinline__ void gather_load( const GatherArgs& a, int64_t p,
                                              GatherIn<Real, MODE, FORMS...>& in )
 {
@@ -705,28 +705,12 @@ __device__ __forceinline__ void gather_load( const GatherArgs& a, int64_t p,
         constexpr int t = I;
         constexpr int NP = FormTraits<GN::f[t]>::NP;
         if constexpr ( mode_out( MODE ) )
-        {
-            const PFieldDev& o = a.t[t].pout;
-            const uint32_t b = o.base( p );
-#pragma unroll
-            for ( int c = 0; c < NP; ++c )
-                in.old[t][c] = *o.template ptr<const Real>( b, c );
-        }
+            a.t[t].pout.template load<Real, NP, CONTIG>( p, in.old[t] );
         if constexpr ( mode_xbar( MODE ) )
-        {
-            const uint32_t b = a.t[t].px.base( p );
-#pragma unroll
-            for ( int c = 0; c < NP; ++c )
-                in.P[t][c] = *a.t[t].px.template ptr<const Real>( b, c );
-        }
+            a.t[t].px.template load<Real, NP, CONTIG>( p, in.P[t] );
     } );
     if constexpr ( mode_xbar( MODE ) )
-    {
-        const uint32_t b = a.xbar.base( p );
-#pragma unroll
-        for ( int e = 0; e < 3; ++e )
-            in.xb[e] = *a.xbar.ptr<const double>( b, e );
-    }
+        a.xbar.load<double, 3, CONTIG>( p, in.xb );
 }
 
 // One particle of a gather pass, from its global reads `in`: for each grid
@@ -736,7 +720,7 @@ __device__ __forceinline__ void gather_load( const GatherArgs& a, int64_t p,
 // Slots are finished one after the other, so only one slot's jets are live.
 // Load(slot_constant, i, j, k, f) fills the slot's NG values of stencil node
 // (i, j, k).
-template <int ORDER, class Real, int MODE, bool SHARED, int... FORMS, class Load>
+template <int ORDER, class Real, int MODE, bool CONTIG, bool SHARED, int... FORMS, class Load>
 __device__ __forceinline__ void gather_compute(
     const GatherArgs& a, int64_t p, const Spline3D<ORDER, Real, mode_xbar( MODE )>& s,
     Load&& load, const GatherIn<Real, MODE, FORMS...>& in )
@@ -769,13 +753,12 @@ __device__ __forceinline__ void gather_compute(
                 const Real m = (Real)a.t[t].mult;
                 if constexpr ( OUT )
                 {
-                    const PFieldDev& o = a.t[t].pout;
                     Real out[NP];
                     jet_dP<F>( J, out );
-                    const uint32_t b = o.base( p );
 #pragma unroll
                     for ( int c = 0; c < NP; ++c )
-                        *o.template ptr<Real>( b, c ) = in.old[t][c] + m * out[c];
+                        out[c] = in.old[t][c] + m * out[c];
+                    a.t[t].pout.template store<Real, NP, CONTIG>( p, out );
                 }
                 if constexpr ( XBAR )
                 {
@@ -791,12 +774,7 @@ __device__ __forceinline__ void gather_compute(
         __builtin_amdgcn_sched_barrier( 0 );
     } );
     if constexpr ( XBAR )
-    {
-        const uint32_t b = a.xbar.base( p );
-#pragma unroll
-        for ( int e = 0; e < 3; ++e )
-            *a.xbar.ptr<double>( b, e ) = xb[e];
-    }
+        a.xbar.store<double, 3, CONTIG>( p, xb );
 }
 
 // One particle of a gather pass: its global reads (issued before the stencil
@@ -807,8 +785,8 @@ __device__ __forceinline__ void gather_particle(
     Load&& load )
 {
     GatherIn<Real, MODE, FORMS...> in;
-    gather_load<Real, MODE, FORMS...>( a, p, in );
-    gather_compute<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, load, in );
+    gather_load<Real, MODE, false, FORMS...>( a, p, in );
+    gather_compute<ORDER, Real, MODE, false, SHARED, FORMS...>( a, p, s, load, in );
 }
 
 // Loader of stencil nodes straight from the grid arrays.
@@ -838,13 +816,11 @@ __device__ __forceinline__ void atomic_add( Real* p, Real v )
     unsafeAtomicAdd( p, v );
 }
 
+template <bool CONTIG = false>
 __device__ __forceinline__ void load_position( const PFieldDev& f, int64_t p,
                                                double px[3] )
 {
-    const uint32_t b = f.base( p );
-    px[0] = *f.ptr<double>( b, 0 );
-    px[1] = *f.ptr<double>( b, 1 );
-    px[2] = *f.ptr<double>( b, 2 );
+    f.load<double, 3, CONTIG>( p, px );
 }
 
 } // namespace cabana_amd

@@ -53,7 +53,121 @@ struct PFieldDev
     {
         return reinterpret_cast<T*>( data + (uint64_t)( b + vl * (uint32_t)c ) * sizeof( T ) );
     }
+    // Components [0, N) of particle p.  CONTIG (vector_length 1, checked
+    // by the caller's host code) moves them as 16/8-byte vectors (4-byte
+    // aligned access), one instruction per 16 bytes instead of one per
+    // component; otherwise (AoSoA) each component is its own access,
+    // coalesced across lanes.
+    template <class T, int N, bool CONTIG = false>
+    __device__ __forceinline__ void load( int64_t p, T* v ) const;
+    template <class T, int N, bool CONTIG = false>
+    __device__ __forceinline__ void store( int64_t p, const T* v ) const;
 };
+
+namespace detail
+{
+template <class T, int W>
+struct VecOf;
+template <>
+struct VecOf<float, 4>
+{
+    typedef float type __attribute__( ( ext_vector_type( 4 ), aligned( 4 ) ) );
+};
+template <>
+struct VecOf<float, 2>
+{
+    typedef float type __attribute__( ( ext_vector_type( 2 ), aligned( 4 ) ) );
+};
+template <>
+struct VecOf<double, 2>
+{
+    typedef double type __attribute__( ( ext_vector_type( 2 ), aligned( 8 ) ) );
+};
+
+template <class T, int N>
+__device__ __forceinline__ void vec_copy_in( const T* src, T* dst )
+{
+    constexpr int W = 16 / (int)sizeof( T );
+    int i = 0;
+#pragma unroll
+    for ( ; i + W <= N; i += W )
+    {
+        const typename VecOf<T, W>::type x = *reinterpret_cast<const typename VecOf<T, W>::type*>( src + i );
+#pragma unroll
+        for ( int e = 0; e < W; ++e )
+            dst[i + e] = x[e];
+    }
+    if constexpr ( W == 4 && N % 4 >= 2 )
+    {
+        const typename VecOf<T, 2>::type x = *reinterpret_cast<const typename VecOf<T, 2>::type*>( src + i );
+        dst[i] = x[0];
+        dst[i + 1] = x[1];
+        i += 2;
+    }
+#pragma unroll
+    for ( ; i < N; ++i )
+        dst[i] = src[i];
+}
+
+template <class T, int N>
+__device__ __forceinline__ void vec_copy_out( const T* src, T* dst )
+{
+    constexpr int W = 16 / (int)sizeof( T );
+    int i = 0;
+#pragma unroll
+    for ( ; i + W <= N; i += W )
+    {
+        typename VecOf<T, W>::type x;
+#pragma unroll
+        for ( int e = 0; e < W; ++e )
+            x[e] = src[i + e];
+        *reinterpret_cast<typename VecOf<T, W>::type*>( dst + i ) = x;
+    }
+    if constexpr ( W == 4 && N % 4 >= 2 )
+    {
+        typename VecOf<T, 2>::type x;
+        x[0] = src[i];
+        x[1] = src[i + 1];
+        *reinterpret_cast<typename VecOf<T, 2>::type*>( dst + i ) = x;
+        i += 2;
+    }
+#pragma unroll
+    for ( ; i < N; ++i )
+        dst[i] = src[i];
+}
+} // namespace detail
+
+template <class T, int N, bool CONTIG>
+__device__ __forceinline__ void PFieldDev::load( int64_t p, T* v ) const
+{
+    const uint32_t b = base( p );
+    if constexpr ( CONTIG )
+    {
+        detail::vec_copy_in<T, N>( ptr<const T>( b, 0 ), v );
+    }
+    else
+    {
+#pragma unroll
+        for ( int c = 0; c < N; ++c )
+            v[c] = *ptr<const T>( b, c );
+    }
+}
+
+template <class T, int N, bool CONTIG>
+__device__ __forceinline__ void PFieldDev::store( int64_t p, const T* v ) const
+{
+    const uint32_t b = base( p );
+    if constexpr ( CONTIG )
+    {
+        detail::vec_copy_out<T, N>( v, ptr<T>( b, 0 ) );
+    }
+    else
+    {
+#pragma unroll
+        for ( int c = 0; c < N; ++c )
+            *ptr<T>( b, c ) = v[c];
+    }
+}
 
 // Grid array: element strides are 32-bit and every element's byte offset fits
 // in 32 bits (checked on the host), so device addressing is a uniform 64-bit
@@ -147,6 +261,7 @@ struct ScatterArgs
     int64_t end;
     int32_t nterms;
     int32_t has_xbar;
+    int32_t contig; // every particle field of the pass has vector_length 1
     ScatterTermDev t[MAX_TERMS];
     unsigned long long* skipped;
 };
@@ -160,6 +275,7 @@ struct GatherArgs
     int64_t end;
     int32_t nterms;
     int32_t has_xbar;
+    int32_t contig; // every particle field of the pass has vector_length 1
     GatherTermDev t[MAX_TERMS];
     unsigned long long* skipped;
 };

@@ -204,7 +204,7 @@ __device__ __forceinline__ void window_coords( const TileArgs& ta, int f,
 // Tiled gather
 // ---------------------------------------------------------------------------
 
-template <class Real, int MODE, bool SHARED, int... FORMS>
+template <class Real, int MODE, bool CONTIG, bool SHARED, int... FORMS>
 __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
     GatherArgs a, TileArgs ta )
 {
@@ -287,8 +287,8 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
         {
             r = find_run( run_pre, nruns, v );
             p = ta.bin_begin + run_start[r] + ( v - run_pre[r] );
-            load_position( a.points, p, px );
-            gather_load<Real, MODE, FORMS...>( a, p, in );
+            load_position<CONTIG>( a.points, p, px );
+            gather_load<Real, MODE, CONTIG, FORMS...>( a, p, in );
         }
         while ( v < pend )
         {
@@ -301,8 +301,8 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
                 while ( run_pre[r + 1] <= vn )
                     ++r;
                 pn = ta.bin_begin + run_start[r] + ( vn - run_pre[r] );
-                load_position( a.points, pn, pxn );
-                gather_load<Real, MODE, FORMS...>( a, pn, inn );
+                load_position<CONTIG>( a.points, pn, pxn );
+                gather_load<Real, MODE, CONTIG, FORMS...>( a, pn, inn );
             }
             Spline3D<ORDER, Real, mode_xbar( MODE )> s;
             if ( !eval_spline( a.mesh, px, s ) )
@@ -344,13 +344,13 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
                             f[2] = q[2];
                         }
                     };
-                    gather_compute<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, ld, in );
+                    gather_compute<ORDER, Real, MODE, CONTIG, SHARED, FORMS...>( a, p, s, ld, in );
                 }
                 else
                 {
                     GlobalGatherLoad<Real, SHARED, FORMS...> ld{
                         a, { s.d[0].base, s.d[1].base, s.d[2].base } };
-                    gather_compute<ORDER, Real, MODE, SHARED, FORMS...>( a, p, s, ld, in );
+                    gather_compute<ORDER, Real, MODE, CONTIG, SHARED, FORMS...>( a, p, s, ld, in );
                 }
             }
             v = vn;
@@ -551,7 +551,7 @@ __device__ __forceinline__ unsigned pack_task( int base, int start, int len )
     return ( (unsigned)base << 20 ) | ( (unsigned)start << 5 ) | (unsigned)len;
 }
 
-template <class Real, bool SHARED, int... FORMS>
+template <class Real, bool CONTIG, bool SHARED, int... FORMS>
 __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
     ScatterArgs a, TileArgs ta )
 {
@@ -629,7 +629,7 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
                 continue;
             const int64_t p = run_particle( ta, run_start, run_pre, nruns, pbeg + i );
             double px[3];
-            load_position( a.points, p, px );
+            load_position<CONTIG>( a.points, p, px );
             int bs[3];
             bool ok = true;
 #pragma unroll
@@ -659,11 +659,11 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
                     constexpr int t = I;
                     constexpr int NP = np_of( FL::v[t] );
                     const Real m = (Real)a.t[t].mult;
-                    const uint32_t b = a.t[t].pin.base( p );
+                    Real v[NP];
+                    a.t[t].pin.template load<Real, NP, CONTIG>( p, v );
 #pragma unroll
                     for ( int c = 0; c < NP; ++c )
-                        stash[mi][RL::attr( t ) + c] =
-                            *a.t[t].pin.template ptr<const Real>( b, c ) * m;
+                        stash[mi][RL::attr( t ) + c] = v[c] * m;
                 } );
             }
             else
@@ -899,31 +899,25 @@ hipError_t launch_tile_kernel( KernelPtr k, const Args& a, const TileArgs& t,
     return hipGetLastError();
 }
 
-template <class Real>
-hipError_t gather_tiled_typed( Launcher& L, int plan, const GatherArgs& a,
-                               const TileArgs& t, size_t lds, hipStream_t s )
+template <class Real, bool CONTIG>
+hipError_t gather_tiled_launch( int plan, int mode, const GatherArgs& a, const TileArgs& t,
+                                size_t lds, hipStream_t s )
 {
-    const int mode = !a.has_xbar ? GATHER_OUT : ( a.t[0].pout.data ? GATHER_OUT_XBAR : GATHER_XBAR );
-    const char* name =
-        a.has_xbar ? ( plan == PLAN_MPM_P2G ? "p2g_adjoint_mpm_tiled" : "gather_xbar_tiled" )
-                   : ( plan == PLAN_MPM_G2P ? "g2p_mpm_tiled" : "g2p_tiled" );
-    hipError_t e = hipSuccess;
-    L.before( name, s );
     if ( plan == PLAN_SINGLE )
     {
         switch ( a.t[0].form )
         {
 #define CA_CASE( F )                                                           \
     case F:                                                                    \
-        e = mode == GATHER_OUT                                                 \
-                ? launch_tile_kernel( gather_tiled_kernel<Real, GATHER_OUT, false, F>, a, \
-                                      t, lds, s )                              \
-            : mode == GATHER_OUT_XBAR                                          \
-                ? launch_tile_kernel( gather_tiled_kernel<Real, GATHER_OUT_XBAR, false, F>, \
-                                      a, t, lds, s )                           \
-                : launch_tile_kernel( gather_tiled_kernel<Real, GATHER_XBAR, false, F>, a, \
-                                      t, lds, s );                             \
-        break;
+        return mode == GATHER_OUT                                              \
+                   ? launch_tile_kernel( gather_tiled_kernel<Real, GATHER_OUT, CONTIG, false, F>, \
+                                         a, t, lds, s )                        \
+               : mode == GATHER_OUT_XBAR                                       \
+                   ? launch_tile_kernel(                                       \
+                         gather_tiled_kernel<Real, GATHER_OUT_XBAR, CONTIG, false, F>, a, t, \
+                         lds, s )                                              \
+                   : launch_tile_kernel( gather_tiled_kernel<Real, GATHER_XBAR, CONTIG, false, F>, \
+                                         a, t, lds, s );
             CA_CASE( FORM_SV )
             CA_CASE( FORM_VV )
             CA_CASE( FORM_SG )
@@ -932,23 +926,66 @@ hipError_t gather_tiled_typed( Launcher& L, int plan, const GatherArgs& a,
             CA_CASE( FORM_TDT )
 #undef CA_CASE
         default:
-            e = hipErrorInvalidValue;
+            return hipErrorInvalidValue;
         }
     }
-    else if ( plan == PLAN_MPM_G2P && mode == GATHER_OUT )
-        e = launch_tile_kernel( gather_tiled_kernel<Real, GATHER_OUT, true, FORM_VV, FORM_TDT>,
-                                a, t, lds, s );
-    else if ( plan == PLAN_MPM_G2P && mode == GATHER_XBAR )
-        e = launch_tile_kernel( gather_tiled_kernel<Real, GATHER_XBAR, true, FORM_VV, FORM_TDT>,
-                                a, t, lds, s );
-    else if ( plan == PLAN_MPM_P2G && mode == GATHER_OUT_XBAR )
-        e = launch_tile_kernel(
-            gather_tiled_kernel<Real, GATHER_OUT_XBAR, false, FORM_SV, FORM_VV, FORM_TD>, a,
-            t, lds, s );
-    else
-        e = hipErrorInvalidValue;
+    if ( plan == PLAN_MPM_G2P && mode == GATHER_OUT )
+        return launch_tile_kernel(
+            gather_tiled_kernel<Real, GATHER_OUT, CONTIG, true, FORM_VV, FORM_TDT>, a, t, lds, s );
+    if ( plan == PLAN_MPM_G2P && mode == GATHER_XBAR )
+        return launch_tile_kernel(
+            gather_tiled_kernel<Real, GATHER_XBAR, CONTIG, true, FORM_VV, FORM_TDT>, a, t, lds, s );
+    if ( plan == PLAN_MPM_P2G && mode == GATHER_OUT_XBAR )
+        return launch_tile_kernel(
+            gather_tiled_kernel<Real, GATHER_OUT_XBAR, CONTIG, false, FORM_SV, FORM_VV, FORM_TD>,
+            a, t, lds, s );
+    return hipErrorInvalidValue;
+}
+
+template <class Real>
+hipError_t gather_tiled_typed( Launcher& L, int plan, const GatherArgs& a,
+                               const TileArgs& t, size_t lds, hipStream_t s )
+{
+    const int mode = !a.has_xbar ? GATHER_OUT : ( a.t[0].pout.data ? GATHER_OUT_XBAR : GATHER_XBAR );
+    const char* name =
+        a.has_xbar ? ( plan == PLAN_MPM_P2G ? "p2g_adjoint_mpm_tiled" : "gather_xbar_tiled" )
+                   : ( plan == PLAN_MPM_G2P ? "g2p_mpm_tiled" : "g2p_tiled" );
+    L.before( name, s );
+    const hipError_t e = a.contig ? gather_tiled_launch<Real, true>( plan, mode, a, t, lds, s )
+                                  : gather_tiled_launch<Real, false>( plan, mode, a, t, lds, s );
     L.after( name, s );
     return e;
+}
+
+template <class Real, bool CONTIG>
+hipError_t scatter_cells_launch( int plan, const ScatterArgs& a, const TileArgs& t, size_t lds,
+                                 hipStream_t s )
+{
+    if ( plan == PLAN_SINGLE )
+    {
+        switch ( a.t[0].form )
+        {
+#define CA_CASE( F )                                                           \
+    case F:                                                                    \
+        return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, false, F>, a, t, lds, s );
+            CA_CASE( FORM_SV )
+            CA_CASE( FORM_VV )
+            CA_CASE( FORM_SG )
+            CA_CASE( FORM_VD )
+            CA_CASE( FORM_TD )
+            CA_CASE( FORM_TDT )
+#undef CA_CASE
+        default:
+            return hipErrorInvalidValue;
+        }
+    }
+    if ( plan == PLAN_MPM_P2G )
+        return launch_tile_kernel(
+            scatter_cells_kernel<Real, CONTIG, false, FORM_SV, FORM_VV, FORM_TD>, a, t, lds, s );
+    if ( plan == PLAN_MPM_G2P )
+        return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, true, FORM_VV, FORM_TDT>,
+                                   a, t, lds, s );
+    return hipErrorInvalidValue;
 }
 
 template <class Real>
@@ -958,35 +995,9 @@ hipError_t scatter_cells_typed( Launcher& L, int plan, const ScatterArgs& a,
     const char* name = plan == PLAN_MPM_P2G   ? "p2g_mpm_cells"
                        : plan == PLAN_MPM_G2P ? "g2p_adjoint_mpm_cells"
                                               : "scatter_cells";
-    hipError_t e = hipSuccess;
     L.before( name, s );
-    if ( plan == PLAN_SINGLE )
-    {
-        switch ( a.t[0].form )
-        {
-#define CA_CASE( F )                                                           \
-    case F:                                                                    \
-        e = launch_tile_kernel( scatter_cells_kernel<Real, false, F>, a, t, lds, s ); \
-        break;
-            CA_CASE( FORM_SV )
-            CA_CASE( FORM_VV )
-            CA_CASE( FORM_SG )
-            CA_CASE( FORM_VD )
-            CA_CASE( FORM_TD )
-            CA_CASE( FORM_TDT )
-#undef CA_CASE
-        default:
-            e = hipErrorInvalidValue;
-        }
-    }
-    else if ( plan == PLAN_MPM_P2G )
-        e = launch_tile_kernel(
-            scatter_cells_kernel<Real, false, FORM_SV, FORM_VV, FORM_TD>, a, t, lds, s );
-    else if ( plan == PLAN_MPM_G2P )
-        e = launch_tile_kernel( scatter_cells_kernel<Real, true, FORM_VV, FORM_TDT>, a,
-                                t, lds, s );
-    else
-        e = hipErrorInvalidValue;
+    const hipError_t e = a.contig ? scatter_cells_launch<Real, true>( plan, a, t, lds, s )
+                                  : scatter_cells_launch<Real, false>( plan, a, t, lds, s );
     L.after( name, s );
     return e;
 }
