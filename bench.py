@@ -147,11 +147,74 @@ class Workload:
                        [self.gbar_m, self.gbar_q, self.gbar_s],
                        [self.mbar, self.qbar, self.sbar], self.xbar, bins=self.lcl)
 
+    def outputs_of_one_step(self):
+        """Every output of one step from zeroed accumulators, as FP64 device
+        tensors (owned grid nodes only)."""
+        torch = self.torch
+        for t in (self.vel, self.L, self.xbar, self.mbar, self.qbar, self.sbar):
+            t.zero_()
+        self.step()
+        own = self.lg.indexSpace(self.ca.Own(), self.ca.Node())
+        sl = tuple(slice(own.min(d), own.max(d)) for d in range(3))
+        g = lambda a: a.view()[sl].double().clone()  # noqa: E731
+        out = {"p2g_mass": g(self.gmass), "p2g_momentum": g(self.gmom),
+               "p2g_stress_div": g(self.gstress), "g2p_velocity": self.vel.double().clone(),
+               "g2p_velocity_grad": self.L.double().clone(),
+               "g2p_adjoint_grid": g(self.gbar_u), "adjoint_positions": self.xbar.clone(),
+               "p2g_adjoint_mass": self.mbar.double().clone(),
+               "p2g_adjoint_momentum": self.qbar.double().clone(),
+               "p2g_adjoint_stress": self.sbar.double().clone()}
+        torch.cuda.synchronize()
+        return out
+
     def sort_step(self):
         """Per-step re-binning: LinkedCellList build + permute of the
         positions and the 13 particle attributes."""
         self.lcl.build(self.pos, 0, self.n)
         self.ca.permute(self.lcl, self.pos, self.m, self.q, self.sig)
+
+
+def field_errors(torch, got, ref):
+    out = {}
+    for k in ref:
+        a, b = got[k], ref[k]
+        nb = torch.linalg.norm(b).item()
+        out[k] = {"rel_l2": torch.linalg.norm(a - b).item() / (nb if nb > 0 else 1.0),
+                  "max_abs": (a - b).abs().max().item(),
+                  "max_ref": b.abs().max().item()}
+    return out
+
+
+def small_precision_figure(torch, ca, order, cells=32, ppc=8, seed=303):
+    """32^3 periodic cells x 8 ppc: ScalarValueP2G of mass and VectorValueG2P
+    of u_d = sin(2 pi x_d / L), FP32-mixed against FP64 on the same inputs."""
+    dev = torch.device("cuda", 0)
+    n = cells ** 3 * ppc
+    L = float(cells)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    pos = torch.rand(n, 3, dtype=torch.float64, device=dev, generator=g) * L
+    mass = torch.rand(n, dtype=torch.float64, device=dev, generator=g) + 0.5
+    gm = ca.createUniformGlobalMesh([0.0] * 3, [L] * 3, 1.0)
+    lg = ca.createLocalGrid(ca.createGlobalGrid(gm, [True] * 3), 1)
+    own = lg.indexSpace(ca.Own(), ca.Node())
+    sl = tuple(slice(own.min(d), own.max(d)) for d in range(3))
+    res = {}
+    for dt in (torch.float64, torch.float32):
+        rho = ca.createArray("rho", ca.createArrayLayout(lg, 1, ca.Node()), dtype=dt)
+        u = ca.createArray("u", ca.createArrayLayout(lg, 3, ca.Node()), dtype=dt)
+        x = torch.arange(cells + 3, dtype=torch.float64, device=dev) - 1.0
+        sn = torch.sin(2 * math.pi * x / L)
+        uv = u.view()
+        uv[..., 0] = sn[:, None, None]
+        uv[..., 1] = sn[None, :, None]
+        uv[..., 2] = sn[None, None, :]
+        vel = torch.zeros(n, 3, dtype=dt, device=dev)
+        halo = ca.createHalo(ca.NodeHaloPattern(), 1, rho)
+        ca.p2g(ca.createScalarValueP2G(mass.to(dt), 1.0), pos, n, ca.Spline(order), halo, rho)
+        ca.g2p(u, halo, pos, n, ca.Spline(order), ca.createVectorValueG2P(vel, 1.0))
+        res[dt] = {"p2g_mass": rho.view()[sl].double(), "g2p_velocity": vel.double()}
+    return field_errors(torch, res[torch.float32], res[torch.float64])
 
 
 def differentiation_figure(torch, ca, steps=5, warmup=2, cells=64, ppc=8, seed=404):
@@ -350,6 +413,7 @@ def main():
         torch.cuda.synchronize()
         del x, y
     res = {}
+    outputs = {}
     modes = [("fp32", torch.float32), ("fp64", torch.float64)]
     if args.mode != "both":
         modes = [m for m in modes if m[0] == args.mode]
@@ -358,6 +422,7 @@ def main():
         ms = time_steps(torch, w.step, args.steps, args.warmup)
         psteps = max(2, min(args.steps, 5))
         prof = kernel_profile(_native, torch, w.step, psteps)
+        outputs[label] = w.outputs_of_one_step()
         ms_sort = time_steps(torch, w.sort_step, 2, 1)
         ca.check_domain()
         res[label] = {"ms": ms, "rate": w.n / (ms * 1e-3), "kernels": prof,
@@ -384,6 +449,16 @@ def main():
     roofline["step_algorithmic_bytes"] = step_bytes
     roofline["step_frac"] = step_bytes / (head["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     extra = {}
+    if "fp32" in outputs and "fp64" in outputs:
+        extra["precision"] = {
+            "reference": "GPU FP64 path (pinned to the oracle at 1e-12 rel-L2 by the tests)",
+            "%d^3x%dppc_order%d" % (args.cells, args.ppc, args.order):
+                field_errors(torch, outputs["fp32"], outputs["fp64"])}
+        if not args.no_extra:
+            for o in (1, 3):
+                extra["precision"]["32^3x8ppc_order%d" % o] = small_precision_figure(torch, ca, o)
+    outputs.clear()
+    torch.cuda.empty_cache()
     if not args.no_extra and "fp32" in res and args.cells == 128:
         extra["differentiation"] = differentiation_figure(torch, ca)
         torch.cuda.empty_cache()

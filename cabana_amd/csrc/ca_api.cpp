@@ -281,7 +281,16 @@ cabana_amd_status make_pfield( cabana_amd_context* ctx,
     out.vl = (uint32_t)f.vector_length;
     out.stride = (uint32_t)f.soa_stride;
     out.ncomp = f.num_comp;
-    if ( f.vector_length >= f.num_particle )
+    if ( f.num_comp == 1 && f.vector_length >= f.num_particle )
+    {
+        // A single-component field inside one vector: element p sits at p,
+        // the same address as vector_length 1 with unit stride.
+        out.vl = 1;
+        out.stride = 1;
+        out.shift = 0;
+        out.mask = 0;
+    }
+    else if ( f.vector_length >= f.num_particle )
     {
         out.shift = 31;
         out.mask = 0x7fffffffu;
@@ -461,7 +470,7 @@ enum TileKind
 };
 
 constexpr int TILE_BLOCK = 256;
-constexpr int SCATTER_SUB_LEN = 8;
+constexpr int SCATTER_SUB_LEN = 12;
 
 struct TileShape
 {
@@ -503,8 +512,11 @@ size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng, int nslot,
     b = a16( b + 4 * ( nbase + 1 ) );
     b = a16( b + 4 * (size_t)sh.max_tasks );
     b = a16( b + real * rs * (size_t)sh.cap );
-    b = a16( b + 8 * runs );
-    b = a16( b + 4 * ( runs + 1 ) );
+    for ( int buf = 0; buf < 2; ++buf )
+    {
+        b = a16( b + 8 * runs );
+        b = a16( b + 4 * ( runs + 1 ) );
+    }
     b += 4 * ( TILE_BLOCK / 64 ) + 16;
     return b;
 }
@@ -613,6 +625,11 @@ bool make_tiles( TileKind kind, const MeshDev& m, int order, bool fp64,
         t.win[d] = sh.T[d] + extra;
     }
     t.max_runs = sh.T[0] * sh.T[1];
+    // The scatter keeps a run table in one wave (<= 64 runs) and base-cell
+    // coordinates in 4-bit task fields.
+    if ( kind == TILE_SCATTER &&
+         ( t.max_runs > 64 || sh.T[0] > 14 || sh.T[1] > 14 || sh.T[2] > 14 ) )
+        return false;
     t.max_win_nodes = t.win[0] * t.win[1] * t.win[2];
     t.max_base_cells = base_cells( sh.T, order );
     t.cap = sh.cap;
@@ -1468,37 +1485,48 @@ extern "C" cabana_amd_status cabana_amd_linked_cell_build(
     if ( !cart_init( grid_delta, grid_min, grid_max, g, ncell ) )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                      "invalid Cartesian grid" );
+    if ( ncell >= INT32_MAX )
+        return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED,
+                     "more than 2^31 - 2 bins" );
     hipStream_t s = static_cast<hipStream_t>( stream );
     const int64_t np = end - begin;
-    const int64_t nblock = ( ncell + 1023 ) / 1024;
-    CA_HIP( ctx, ctx->ensure( SLOT_CELL_OF, sizeof( int32_t ) * ( np + 1 ) ),
+    // Keys are 0..ncell-1 and ncell for particles outside the grid.
+    int key_bits = 0;
+    while ( ( ncell >> key_bits ) != 0 )
+        ++key_bits;
+    int digit_bits;
+    lcl_sort_passes( key_bits, &digit_bits );
+    const int64_t nhist = ( (int64_t)1 << digit_bits ) * lcl_sort_tiles( np );
+    const int64_t nblock = ( nhist + 1023 ) / 1024 + 1;
+    const int64_t nkey = np + 64;
+    CA_HIP( ctx, ctx->ensure( SLOT_CELL_OF, sizeof( int32_t ) * 4 * nkey ),
             "scratch" );
-    CA_HIP( ctx, ctx->ensure( SLOT_CURSOR, sizeof( int64_t ) * ( ncell + 1 ) ),
+    CA_HIP( ctx,
+            ctx->ensure( SLOT_CURSOR,
+                         ( sizeof( int64_t ) + sizeof( int32_t ) ) * ( nhist + 64 ) ),
             "scratch" );
-    CA_HIP( ctx, ctx->ensure( SLOT_SCAN, sizeof( int64_t ) * ( nblock + 1 ) ),
+    CA_HIP( ctx, ctx->ensure( SLOT_SCAN, sizeof( int64_t ) * nblock ),
             "scratch" );
-    int32_t* cell_of = static_cast<int32_t*>( ctx->scratch[SLOT_CELL_OF] );
-    int64_t* cursor = static_cast<int64_t*>( ctx->scratch[SLOT_CURSOR] );
+    int32_t* keys = static_cast<int32_t*>( ctx->scratch[SLOT_CELL_OF] );
+    int64_t* digit_base = static_cast<int64_t*>( ctx->scratch[SLOT_CURSOR] );
+    int32_t* hist = reinterpret_cast<int32_t*>( digit_base + nhist + 64 );
     int64_t* sums = static_cast<int64_t*>( ctx->scratch[SLOT_SCAN] );
     CA_HIP( ctx,
             launch_fill_i32( *ctx, reinterpret_cast<int32_t*>( ctx->bad ), 2, 0,
                              s ),
             "fill" );
-    CA_HIP( ctx, launch_fill_i32( *ctx, counts, ncell, 0, s ), "fill" );
     CA_HIP( ctx,
-            launch_lcl_count( *ctx, pos, begin, end, g, counts, cell_of,
-                              ctx->bad, s ),
-            "count" );
+            launch_lcl_keys( *ctx, pos, begin, end, g, (int32_t)ncell, keys,
+                             ctx->bad, s ),
+            "bin keys" );
+    int32_t* sorted = keys;
     CA_HIP( ctx,
-            launch_exclusive_scan( *ctx, counts, offsets, ncell, sums,
-                                   nblock + 1, s ),
-            "scan" );
-    CA_HIP( ctx, launch_copy_i64( *ctx, offsets, cursor, ncell, s ), "copy" );
-    CA_HIP( ctx,
-            launch_lcl_place( *ctx, begin, end, cell_of, cursor, permutes, s ),
-            "place" );
-    CA_HIP( ctx, launch_lcl_order_bins( *ctx, offsets, counts, ncell, permutes, s ),
-            "order bins" );
+            launch_lcl_sort( *ctx, np, key_bits, keys, keys + nkey,
+                             keys + 2 * nkey, keys + 3 * nkey, hist, digit_base,
+                             sums, nblock, permutes, begin, &sorted, s ),
+            "bin sort" );
+    CA_HIP( ctx, launch_lcl_bounds( *ctx, sorted, np, ncell, counts, offsets, s ),
+            "bin bounds" );
     // The reference fences after each phase; one wait here reports
     // particles outside the grid.
     unsigned long long bad = 0;

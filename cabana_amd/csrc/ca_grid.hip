@@ -187,9 +187,12 @@ __device__ __forceinline__ int64_t locate_cell( const CartGridDev& g,
     return ( (int64_t)c[0] * g.nx[1] + c[1] ) * g.nx[2] + c[2];
 }
 
-__global__ __launch_bounds__( BLOCK ) void lcl_count_kernel(
-    PFieldDev pos, int64_t begin, int64_t end, CartGridDev g,
-    int32_t* counts, int32_t* cell_of, unsigned long long* bad )
+// Bin key of every particle in [begin, end): its cardinal cell, or bad_key
+// (one past the last cell) when the particle lies outside the grid.  No
+// per-cell counters are touched here: the counts come out of the sort.
+__global__ __launch_bounds__( BLOCK ) void lcl_key_kernel(
+    PFieldDev pos, int64_t begin, int64_t end, CartGridDev g, int32_t bad_key,
+    int32_t* keys, unsigned long long* bad )
 {
     const int64_t p = begin + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if ( p >= end )
@@ -199,14 +202,10 @@ __global__ __launch_bounds__( BLOCK ) void lcl_count_kernel(
                           *pos.ptr<const double>( b, 1 ),
                           *pos.ptr<const double>( b, 2 ) };
     const int64_t c = locate_cell( g, x );
-    if ( c < 0 )
-    {
-        atomicAdd( bad, 1ull );
-        cell_of[p - begin] = -1;
-        return;
-    }
-    cell_of[p - begin] = (int32_t)c;
-    atomicAdd( counts + c, 1 );
+    keys[p - begin] = c < 0 ? bad_key : (int32_t)c;
+    const unsigned long long out = __ballot( c < 0 );
+    if ( out && (int)( threadIdx.x & 63 ) == __ffsll( (long long)out ) - 1 )
+        atomicAdd( bad, (unsigned long long)__popcll( out ) );
 }
 
 constexpr int SCAN_BLOCK = 1024;
@@ -298,45 +297,218 @@ __global__ __launch_bounds__( SCAN_BLOCK ) void scan_apply_kernel(
         out[i] = block_sums[blockIdx.x] + wsum[wv] + v - x;
 }
 
-__global__ __launch_bounds__( BLOCK ) void lcl_place_kernel(
-    int64_t begin, int64_t end, const int32_t* cell_of, int64_t* cursor,
-    int64_t* permutes )
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of the bin keys.  Cabana's Serial build leaves the
+// particles of one bin in ascending index order
+// (Cabana_LinkedCellList.hpp:710-731, one thread); a stable sort by key gives
+// that order directly, with no per-bin atomics and no per-bin sorting, so
+// crowded bins cost the same as sparse ones.
+//
+// A tile of RS_TILE keys belongs to one workgroup; wave w owns the contiguous
+// run [w * 64 * RS_ITEMS, (w + 1) * 64 * RS_ITEMS) of its tile and walks it
+// 64 keys at a time, so (wave, step, lane) is index order.  Pass layout:
+//   hist:    per-tile digit counts, digit-major: hist[digit * ntiles + tile];
+//   scan:    exclusive scan of hist gives every (digit, tile) its output base;
+//   scatter: each key's rank among equal digits of its tile (ballot peers +
+//            per-wave running counts in LDS), a tile-local reorder in LDS,
+//            then writes in tile order, so equal digits leave contiguously.
+// ---------------------------------------------------------------------------
+constexpr int RS_BLOCK = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
+constexpr int RS_WAVES = RS_BLOCK / 64;
+constexpr int RS_MAX_BITS = 8;
+constexpr int RS_MAX_BINS = 1 << RS_MAX_BITS;
+static_assert( RS_MAX_BINS == RS_BLOCK, "one digit per thread in the scans" );
+
+__global__ __launch_bounds__( RS_BLOCK ) void radix_hist_kernel(
+    const int32_t* keys, int64_t n, int shift, int nbins, int64_t ntiles,
+    int32_t* hist )
 {
-    const int64_t p = begin + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if ( p >= end )
-        return;
-    const int32_t c = cell_of[p - begin];
-    if ( c < 0 )
-        return;
-    const unsigned long long slot = atomicAdd(
-        reinterpret_cast<unsigned long long*>( cursor + c ), 1ull );
-    permutes[slot] = p;
+    __shared__ uint32_t h[RS_MAX_BINS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+    const uint32_t mask = (uint32_t)nbins - 1u;
+#pragma unroll 4
+    for ( int k = 0; k < RS_ITEMS; ++k )
+    {
+        const int64_t i = t0 + k * RS_BLOCK + threadIdx.x;
+        if ( i < n )
+            atomicAdd( &h[( (uint32_t)keys[i] >> shift ) & mask], 1u );
+    }
+    __syncthreads();
+    if ( (int)threadIdx.x < nbins )
+        hist[threadIdx.x * ntiles + blockIdx.x] = (int32_t)h[threadIdx.x];
 }
 
-// Restores ascending particle order inside every bin (the order a Serial
-// build produces), after the atomic placement above.
-__global__ __launch_bounds__( BLOCK ) void lcl_order_bins_kernel(
-    const int64_t* offsets, const int32_t* counts, int64_t ncell,
-    int64_t* permutes )
+// Exclusive scan of one value per thread over the workgroup; returns the
+// prefix of this thread and leaves the total in *total.
+__device__ __forceinline__ uint32_t rs_block_scan( uint32_t v, uint32_t* wsum,
+                                                   uint32_t* total )
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for ( int o = 1; o < 64; o <<= 1 )
+    {
+        const uint32_t u = __shfl_up( x, o, 64 );
+        if ( lane >= o )
+            x += u;
+    }
+    if ( lane == 63 )
+        wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for ( int q = 0; q < RS_WAVES; ++q )
+    {
+        const uint32_t s = wsum[q];
+        before += q < w ? s : 0u;
+        all += s;
+    }
+    *total = all;
+    return before + x - v;
+}
+
+// One stable pass over bits [shift, shift + dbits).  FIRST: the values are
+// the key indices themselves (not read).  LAST: the values leave as
+// begin + index into the 64-bit permutation.
+template <bool FIRST, bool LAST>
+__global__ __launch_bounds__( RS_BLOCK ) void radix_scatter_kernel(
+    const int32_t* keys_in, const int32_t* vals_in, int64_t n, int shift,
+    int nbins, int64_t ntiles, const int64_t* digit_base, int32_t* keys_out,
+    int32_t* vals_out, int64_t* perm_out, int64_t begin )
+{
+    __shared__ uint32_t wcnt[RS_WAVES][RS_MAX_BINS];
+    __shared__ uint32_t tstart[RS_MAX_BINS];
+    __shared__ int64_t gadj[RS_MAX_BINS];
+    __shared__ uint32_t wsum[RS_WAVES];
+    __shared__ int32_t lkey[RS_TILE];
+    __shared__ int32_t lval[RS_TILE];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for ( int q = 0; q < RS_WAVES; ++q )
+        wcnt[q][threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t wbase = t0 + (int64_t)w * 64 * RS_ITEMS;
+    const uint32_t mask = (uint32_t)nbins - 1u;
+    const uint64_t below = ( 1ull << lane ) - 1ull;
+    int32_t key[RS_ITEMS], val[RS_ITEMS];
+    uint32_t rank[RS_ITEMS];
+#pragma unroll
+    for ( int k = 0; k < RS_ITEMS; ++k )
+    {
+        const int64_t i = wbase + k * 64 + lane;
+        const bool ok = i < n;
+        key[k] = ok ? keys_in[i] : 0;
+        if constexpr ( FIRST )
+            val[k] = (int32_t)i;
+        else
+            val[k] = ok ? vals_in[i] : 0;
+    }
+#pragma unroll
+    for ( int k = 0; k < RS_ITEMS; ++k )
+    {
+        const bool ok = wbase + k * 64 + lane < n;
+        const uint32_t d = ( (uint32_t)key[k] >> shift ) & mask;
+        // Lanes holding the same digit: intersect the ballots of every bit.
+        uint64_t peers = __ballot( ok );
+#pragma unroll
+        for ( int b = 0; b < RS_MAX_BITS; ++b )
+        {
+            const bool set = ( d >> b ) & 1u;
+            const uint64_t bb = __ballot( set );
+            peers &= set ? bb : ~bb;
+        }
+        if ( ok )
+        {
+            const uint32_t c = wcnt[w][d];
+            rank[k] = c + (uint32_t)__popcll( peers & below );
+            // The highest lane of the group advances the running count; it
+            // writes a value computed from its own read, so the write
+            // follows every lane's read of this step.
+            if ( ( peers >> lane ) == 1ull )
+                wcnt[w][d] = c + (uint32_t)__popcll( peers );
+        }
+    }
+    __syncthreads();
+    // Per digit: prefix over the waves, then the tile's digit starts.
+    {
+        const int d = threadIdx.x;
+        uint32_t run = 0;
+#pragma unroll
+        for ( int q = 0; q < RS_WAVES; ++q )
+        {
+            const uint32_t c = wcnt[q][d];
+            wcnt[q][d] = run;
+            run += c;
+        }
+        uint32_t total;
+        const uint32_t st = rs_block_scan( d < nbins ? run : 0u, wsum, &total );
+        tstart[d] = st;
+        gadj[d] = d < nbins ? digit_base[(int64_t)d * ntiles + blockIdx.x] - st
+                            : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for ( int k = 0; k < RS_ITEMS; ++k )
+    {
+        if ( wbase + k * 64 + lane < n )
+        {
+            const uint32_t d = ( (uint32_t)key[k] >> shift ) & mask;
+            const uint32_t lp = tstart[d] + wcnt[w][d] + rank[k];
+            lkey[lp] = key[k];
+            lval[lp] = val[k];
+        }
+    }
+    __syncthreads();
+    const int cnt = (int)( n - t0 < RS_TILE ? n - t0 : RS_TILE );
+    for ( int j = threadIdx.x; j < cnt; j += RS_BLOCK )
+    {
+        const int32_t k = lkey[j];
+        const int64_t o = gadj[( (uint32_t)k >> shift ) & mask] + j;
+        keys_out[o] = k;
+        if constexpr ( LAST )
+            perm_out[o] = begin + lval[j];
+        else
+            vals_out[o] = lval[j];
+    }
+}
+
+// counts and offsets of every bin from the sorted keys: offsets[c] is the
+// first position whose key is >= c (the exclusive scan of the counts,
+// Cabana_LinkedCellList.hpp:700-705), counts[c] the length of the run.
+__device__ __forceinline__ int64_t lower_bound_i32( const int32_t* a,
+                                                    int64_t n, int32_t v )
+{
+    int64_t lo = 0, len = n;
+    while ( len > 0 )
+    {
+        const int64_t half = len >> 1;
+        if ( a[lo + half] < v )
+        {
+            lo += half + 1;
+            len -= half + 1;
+        }
+        else
+            len = half;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__( BLOCK ) void lcl_bounds_kernel(
+    const int32_t* sorted_keys, int64_t n, int64_t ncell, int32_t* counts,
+    int64_t* offsets )
 {
     const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if ( c >= ncell )
         return;
-    const int n = counts[c];
-    if ( n < 2 )
-        return;
-    int64_t* q = permutes + offsets[c];
-    for ( int a = 1; a < n; ++a )
-    {
-        const int64_t v = q[a];
-        int b = a - 1;
-        while ( b >= 0 && q[b] > v )
-        {
-            q[b + 1] = q[b];
-            --b;
-        }
-        q[b + 1] = v;
-    }
+    const int64_t lo = lower_bound_i32( sorted_keys, n, (int32_t)c );
+    const int64_t hi = lower_bound_i32( sorted_keys, n, (int32_t)( c + 1 ) );
+    offsets[c] = lo;
+    counts[c] = (int32_t)( hi - lo );
 }
 
 template <class T>
@@ -417,15 +589,6 @@ __global__ __launch_bounds__( BLOCK ) void fill_i32_kernel( int32_t* dst,
         dst[i] = value;
 }
 
-__global__ __launch_bounds__( BLOCK ) void copy_i64_kernel( const int64_t* src,
-                                                            int64_t* dst,
-                                                            int64_t n )
-{
-    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if ( i < n )
-        dst[i] = src[i];
-}
-
 } // namespace
 
 hipError_t launch_halo_gather( Launcher& L, bool fp64, const MeshDev& m,
@@ -492,18 +655,111 @@ hipError_t launch_zero_ghosts( Launcher& L, bool fp64, const MeshDev& m,
     return hipGetLastError();
 }
 
-hipError_t launch_lcl_count( Launcher& L, const PFieldDev& pos, int64_t begin,
-                             int64_t end, const CartGridDev& g,
-                             int32_t* counts, int32_t* cell_of,
-                             unsigned long long* bad, hipStream_t s )
+hipError_t launch_lcl_keys( Launcher& L, const PFieldDev& pos, int64_t begin,
+                            int64_t end, const CartGridDev& g, int32_t bad_key,
+                            int32_t* keys, unsigned long long* bad,
+                            hipStream_t s )
 {
     if ( end <= begin )
         return hipSuccess;
-    L.before( "lcl_count", s );
-    hipLaunchKernelGGL( lcl_count_kernel, blocks_for( end - begin ),
-                        dim3( BLOCK ), 0, s, pos, begin, end, g, counts,
-                        cell_of, bad );
-    L.after( "lcl_count", s );
+    L.before( "lcl_keys", s );
+    hipLaunchKernelGGL( lcl_key_kernel, blocks_for( end - begin ),
+                        dim3( BLOCK ), 0, s, pos, begin, end, g, bad_key, keys,
+                        bad );
+    L.after( "lcl_keys", s );
+    return hipGetLastError();
+}
+
+int64_t lcl_sort_tiles( int64_t n )
+{
+    return ( n + RS_TILE - 1 ) / RS_TILE;
+}
+
+int lcl_sort_passes( int key_bits, int* digit_bits )
+{
+    const int passes = key_bits <= 0 ? 1 : ( key_bits + RS_MAX_BITS - 1 ) / RS_MAX_BITS;
+    *digit_bits = key_bits <= 0 ? 1 : ( key_bits + passes - 1 ) / passes;
+    return passes;
+}
+
+hipError_t launch_lcl_sort( Launcher& L, int64_t n, int key_bits,
+                            int32_t* keys_a, int32_t* keys_b, int32_t* vals_a,
+                            int32_t* vals_b, int32_t* hist, int64_t* digit_base,
+                            int64_t* block_sums, int64_t nblock_cap,
+                            int64_t* permutes, int64_t begin, int32_t** sorted,
+                            hipStream_t s )
+{
+    *sorted = keys_a;
+    if ( n <= 0 )
+        return hipSuccess;
+    int dbits;
+    const int passes = lcl_sort_passes( key_bits, &dbits );
+    const int nbins = 1 << dbits;
+    const int64_t ntiles = lcl_sort_tiles( n );
+    int32_t* kin = keys_a;
+    int32_t* kout = keys_b;
+    int32_t* vin = vals_a;
+    int32_t* vout = vals_b;
+    for ( int p = 0; p < passes; ++p )
+    {
+        const int shift = p * dbits;
+        L.before( "lcl_sort_hist", s );
+        hipLaunchKernelGGL( radix_hist_kernel, dim3( (unsigned)ntiles ),
+                            dim3( RS_BLOCK ), 0, s, kin, n, shift, nbins,
+                            ntiles, hist );
+        L.after( "lcl_sort_hist", s );
+        hipError_t e = launch_exclusive_scan( L, hist, digit_base,
+                                              (int64_t)nbins * ntiles,
+                                              block_sums, nblock_cap, s );
+        if ( e != hipSuccess )
+            return e;
+        const bool first = p == 0, last = p == passes - 1;
+        L.before( "lcl_sort_scatter", s );
+        if ( first && last )
+            hipLaunchKernelGGL( ( radix_scatter_kernel<true, true> ),
+                                dim3( (unsigned)ntiles ), dim3( RS_BLOCK ), 0,
+                                s, kin, vin, n, shift, nbins, ntiles,
+                                digit_base, kout, vout, permutes, begin );
+        else if ( first )
+            hipLaunchKernelGGL( ( radix_scatter_kernel<true, false> ),
+                                dim3( (unsigned)ntiles ), dim3( RS_BLOCK ), 0,
+                                s, kin, vin, n, shift, nbins, ntiles,
+                                digit_base, kout, vout, permutes, begin );
+        else if ( last )
+            hipLaunchKernelGGL( ( radix_scatter_kernel<false, true> ),
+                                dim3( (unsigned)ntiles ), dim3( RS_BLOCK ), 0,
+                                s, kin, vin, n, shift, nbins, ntiles,
+                                digit_base, kout, vout, permutes, begin );
+        else
+            hipLaunchKernelGGL( ( radix_scatter_kernel<false, false> ),
+                                dim3( (unsigned)ntiles ), dim3( RS_BLOCK ), 0,
+                                s, kin, vin, n, shift, nbins, ntiles,
+                                digit_base, kout, vout, permutes, begin );
+        L.after( "lcl_sort_scatter", s );
+        e = hipGetLastError();
+        if ( e != hipSuccess )
+            return e;
+        int32_t* t = kin;
+        kin = kout;
+        kout = t;
+        t = vin;
+        vin = vout;
+        vout = t;
+    }
+    *sorted = kin;
+    return hipSuccess;
+}
+
+hipError_t launch_lcl_bounds( Launcher& L, const int32_t* sorted_keys,
+                              int64_t n, int64_t ncell, int32_t* counts,
+                              int64_t* offsets, hipStream_t s )
+{
+    if ( ncell <= 0 )
+        return hipSuccess;
+    L.before( "lcl_bounds", s );
+    hipLaunchKernelGGL( lcl_bounds_kernel, blocks_for( ncell ), dim3( BLOCK ),
+                        0, s, sorted_keys, n, ncell, counts, offsets );
+    L.after( "lcl_bounds", s );
     return hipGetLastError();
 }
 
@@ -526,35 +782,6 @@ hipError_t launch_exclusive_scan( Launcher& L, const int32_t* in, int64_t* out,
     L.after( "exclusive_scan", s );
     return hipGetLastError();
 }
-
-hipError_t launch_lcl_place( Launcher& L, int64_t begin, int64_t end,
-                             const int32_t* cell_of, int64_t* cursor,
-                             int64_t* permutes, hipStream_t s )
-{
-    if ( end <= begin )
-        return hipSuccess;
-    L.before( "lcl_place", s );
-    hipLaunchKernelGGL( lcl_place_kernel, blocks_for( end - begin ),
-                        dim3( BLOCK ), 0, s, begin, end, cell_of, cursor,
-                        permutes );
-    L.after( "lcl_place", s );
-    return hipGetLastError();
-}
-
-hipError_t launch_lcl_order_bins( Launcher& L, const int64_t* offsets,
-                                  const int32_t* counts, int64_t ncell,
-                                  int64_t* permutes, hipStream_t s )
-{
-    if ( ncell <= 0 )
-        return hipSuccess;
-    L.before( "lcl_order_bins", s );
-    hipLaunchKernelGGL( lcl_order_bins_kernel, blocks_for( ncell ),
-                        dim3( BLOCK ), 0, s, offsets, counts, ncell,
-                        permutes );
-    L.after( "lcl_order_bins", s );
-    return hipGetLastError();
-}
-
 hipError_t launch_permute_gather( Launcher& L, const PFieldDev& f,
                                   int elem_size, const int64_t* permutes,
                                   int64_t begin, int64_t end, void* scratch,
@@ -610,19 +837,6 @@ hipError_t launch_fill_i32( Launcher& L, int32_t* dst, int64_t n,
     L.after( "fill_i32", s );
     return hipGetLastError();
 }
-
-hipError_t launch_copy_i64( Launcher& L, const int64_t* src, int64_t* dst,
-                            int64_t n, hipStream_t s )
-{
-    if ( n <= 0 )
-        return hipSuccess;
-    L.before( "copy_i64", s );
-    hipLaunchKernelGGL( copy_i64_kernel, blocks_for( n ), dim3( BLOCK ), 0, s,
-                        src, dst, n );
-    L.after( "copy_i64", s );
-    return hipGetLastError();
-}
-
 hipError_t launch_brick_parts( Launcher& L, const TileArgs& t,
                                const int32_t num_cell[3], int32_t* brick_nparts,
                                int64_t* brick_start, int64_t* block_sums,

@@ -384,19 +384,31 @@ struct CartGridDev
     double rdx[3];
     int32_t nx[3];
 };
-hipError_t launch_lcl_count( Launcher& L, const PFieldDev& pos, int64_t begin,
-                             int64_t end, const CartGridDev& g,
-                             int32_t* counts, int32_t* cell_of,
-                             unsigned long long* bad, hipStream_t s );
+// Bin keys (cell index, or bad_key outside the grid) of [begin, end).
+hipError_t launch_lcl_keys( Launcher& L, const PFieldDev& pos, int64_t begin,
+                            int64_t end, const CartGridDev& g, int32_t bad_key,
+                            int32_t* keys, unsigned long long* bad,
+                            hipStream_t s );
 hipError_t launch_exclusive_scan( Launcher& L, const int32_t* in, int64_t* out,
                                   int64_t n, int64_t* block_sums,
                                   int64_t nblock_cap, hipStream_t s );
-hipError_t launch_lcl_place( Launcher& L, int64_t begin, int64_t end,
-                             const int32_t* cell_of, int64_t* cursor,
-                             int64_t* permutes, hipStream_t s );
-hipError_t launch_lcl_order_bins( Launcher& L, const int64_t* offsets,
-                                  const int32_t* counts, int64_t ncell,
-                                  int64_t* permutes, hipStream_t s );
+// Stable radix sort of n keys in keys_a (values: the indices 0..n-1), digit
+// passes over key_bits bits.  Every pass needs hist (int32) and digit_base
+// (int64) of lcl_sort_bins * lcl_sort_tiles entries.  The sorted values
+// leave as begin + index in permutes; *sorted names the buffer holding the
+// sorted keys (keys_a or keys_b).
+int64_t lcl_sort_tiles( int64_t n );
+int lcl_sort_passes( int key_bits, int* digit_bits );
+hipError_t launch_lcl_sort( Launcher& L, int64_t n, int key_bits,
+                            int32_t* keys_a, int32_t* keys_b, int32_t* vals_a,
+                            int32_t* vals_b, int32_t* hist, int64_t* digit_base,
+                            int64_t* block_sums, int64_t nblock_cap,
+                            int64_t* permutes, int64_t begin, int32_t** sorted,
+                            hipStream_t s );
+// counts[c] and offsets[c] of bins 0..ncell-1 from n sorted keys.
+hipError_t launch_lcl_bounds( Launcher& L, const int32_t* sorted_keys,
+                              int64_t n, int64_t ncell, int32_t* counts,
+                              int64_t* offsets, hipStream_t s );
 hipError_t launch_permute_gather( Launcher& L, const PFieldDev& f,
                                   int elem_size, const int64_t* permutes,
                                   int64_t begin, int64_t end, void* scratch,
@@ -405,8 +417,6 @@ hipError_t launch_permute_copy_back( Launcher& L, const PFieldDev& f,
                                      int elem_size, int64_t begin,
                                      int64_t end, const void* scratch,
                                      hipStream_t s );
-hipError_t launch_copy_i64( Launcher& L, const int64_t* src, int64_t* dst,
-                            int64_t n, hipStream_t s );
 hipError_t launch_fill_i32( Launcher& L, int32_t* dst, int64_t n,
                             int32_t value, hipStream_t s );
 // Part table of a brick tiling: brick_nparts (nbricks ints), brick_start

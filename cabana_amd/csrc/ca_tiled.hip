@@ -185,19 +185,29 @@ __device__ __forceinline__ int find_run( const int* run_pre, int nruns, int v )
     return lo;
 }
 
+// floor(a / b) for 0 <= a < 2^22 and b >= 1 from a rounded reciprocal rb of
+// b: (a + 1/2) / b lies at least 1/(2b) from an integer, and the two
+// roundings move it by at most (a + 1/2) / b * 2^-23 < 1/(2b).
+__device__ __forceinline__ int div_small( int a, float rb )
+{
+    return (int)( ( (float)a + 0.5f ) * rb );
+}
+
 // Maps a flat window index to (node, dof) with the grid's unit-stride
-// dimension fastest, so window loads/stores coalesce.
-__device__ __forceinline__ void window_coords( const TileArgs& ta, int f,
-                                               int ng, const int W[3], int q[3],
-                                               int& d )
+// dimension fastest, so window loads/stores coalesce.  rW holds the
+// reciprocals of the window extents (window indices stay below 2^22).
+template <int NG>
+__device__ __forceinline__ void window_coords( const TileArgs& ta, int f, const int W[3],
+                                               const float rW[3], int q[3], int& d )
 {
     const int pa = ta.flush_perm[0], pb = ta.flush_perm[1],
               pc = ta.flush_perm[2];
-    d = f % ng;
-    const int node = f / ng;
-    q[pc] = node % W[pc];
-    q[pb] = ( node / W[pc] ) % W[pb];
-    q[pa] = node / ( W[pc] * W[pb] );
+    const int node = f / NG;
+    d = f - node * NG;
+    const int r = div_small( node, rW[pc] );
+    q[pc] = node - r * W[pc];
+    q[pa] = div_small( r, rW[pb] );
+    q[pb] = r - q[pa] * W[pb];
 }
 
 // ---------------------------------------------------------------------------
@@ -247,6 +257,7 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
             W[d] = ta.win[d];
         }
         const int wnodes = W[0] * W[1] * W[2];
+        const float rW[3] = { 1.0f / (float)W[0], 1.0f / (float)W[1], 1.0f / (float)W[2] };
 
         // Stage the windows (nodes outside the grid read as zero).
         static_for<NSLOT>( [&]( auto I ) {
@@ -258,7 +269,7 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
             for ( int f = threadIdx.x; f < wnodes * NG; f += BLOCK )
             {
                 int q[3], d;
-                window_coords( ta, f, NG, W, q, d );
+                window_coords<NG>( ta, f, W, rW, q, d );
                 const int gi = wlo[0] + q[0], gj = wlo[1] + q[1], gk = wlo[2] + q[2];
                 Real v = Real( 0 );
                 if ( gi >= 0 && gj >= 0 && gk >= 0 && gi < a.mesh.num_node[0] &&
@@ -545,14 +556,300 @@ struct RecordLayout
     static constexpr int len() { return attr( FL::N ); }
 };
 
-// Task-table entry: base cell (12 bits), first record (15 bits), count (5).
-__device__ __forceinline__ unsigned pack_task( int base, int start, int len )
+// Lanes per scatter task: the whole stencil (S = 1), one x-slab (S = K) or
+// one (x, y) pencil (S = K * K), the fewest that keep a lane's register
+// accumulator at <= CA_SLAB_ACC values.
+#ifndef CA_SLAB_ACC
+#define CA_SLAB_ACC 64
+#endif
+#ifndef CA_REC_AHEAD
+#define CA_REC_AHEAD 1
+#endif
+#ifndef CA_PART_AHEAD
+#define CA_PART_AHEAD 1
+#endif
+#ifndef CA_SCATTER_WAVES
+#define CA_SCATTER_WAVES 2
+#endif
+template <int DT>
+struct SlabPlan
 {
-    return ( (unsigned)base << 20 ) | ( (unsigned)start << 5 ) | (unsigned)len;
+    static constexpr int MAXACC = CA_SLAB_ACC;
+    static constexpr int S =
+        ( K * K * K * DT <= MAXACC ) ? 1 : ( ( K * K * DT <= MAXACC ) ? K : K * K );
+    static constexpr int NI = ( S == 1 ) ? K : 1;
+    static constexpr int NJ = ( S == K * K ) ? 1 : K;
+};
+
+// Term groups of a scatter pass: each lane accumulates one group's terms.
+// Value-type terms (weights only) and gradient-type terms (weights and
+// gradients) form separate groups, so neither pays for the other's
+// products; terms that share one grid form a single group.
+template <bool SHARED, int... FORMS>
+struct TermGroups
+{
+    using FL = FormList<FORMS...>;
+    static constexpr bool has( bool grad )
+    {
+        for ( int t = 0; t < FL::N; ++t )
+            if ( form_value_type( FL::v[t] ) != grad )
+                return true;
+        return false;
+    }
+    static constexpr int N = ( SHARED || !has( false ) || !has( true ) ) ? 1 : 2;
+    // Group 1 (when there are two) holds the gradient-type terms.
+    static constexpr bool in( int t, int g )
+    {
+        return N == 1 ? true : ( form_value_type( FL::v[t] ) == ( g == 0 ) );
+    }
+    // Accumulator dofs of group g and term t's offset inside them.
+    static constexpr int dofs( int g )
+    {
+        if ( SHARED )
+            return 3;
+        int o = 0;
+        for ( int t = 0; t < FL::N; ++t )
+            if ( in( t, g ) )
+                o += ng_of( FL::v[t] );
+        return o;
+    }
+    static constexpr int offset( int t, int g )
+    {
+        if ( SHARED )
+            return 0;
+        int o = 0;
+        for ( int i = 0; i < t; ++i )
+            if ( in( i, g ) )
+                o += ng_of( FL::v[i] );
+        return o;
+    }
+    // True when accumulator dof d of group g receives a gradient-type term.
+    static constexpr bool grad_dof( int g, int d )
+    {
+        for ( int t = 0; t < FL::N; ++t )
+        {
+            if ( !in( t, g ) || form_value_type( FL::v[t] ) )
+                continue;
+            const int o = offset( t, g );
+            if ( SHARED || ( d >= o && d < o + ng_of( FL::v[t] ) ) )
+                return true;
+        }
+        return false;
+    }
+    static constexpr bool any_grad( int g )
+    {
+        for ( int t = 0; t < FL::N; ++t )
+            if ( in( t, g ) && !form_value_type( FL::v[t] ) )
+                return true;
+        return false;
+    }
+};
+
+// One record's contribution to a lane's slab accumulator for the terms of
+// group G.  The record holds the particle's three cell offsets and its
+// premultiplied attributes.
+template <class Real, bool SHARED, int G, int S, int... FORMS>
+__device__ __forceinline__ void scatter_record(
+    const Real* rp, const Real rdx[3], int sx, int sy,
+    Real ( *acc )[TermGroups<SHARED, FORMS...>::dofs( G )] )
+{
+    using FL = FormList<FORMS...>;
+    using TG = TermGroups<SHARED, FORMS...>;
+    using RL = RecordLayout<FORMS...>;
+    constexpr int NT = FL::N;
+    constexpr int DG = TG::dofs( G );
+    constexpr int NI = ( S == 1 ) ? K : 1;
+    constexpr int NJ = ( S == K * K ) ? 1 : K;
+    constexpr bool GRAD = TG::any_grad( G );
+    Spline1D<ORDER, Real> sp[3];
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+        spline_weights<ORDER, Real, false>( rp[d], rdx[d], sp[d] );
+    Real wx[NI], gx[NI], wy[NJ], gy[NJ];
+    if constexpr ( S == 1 )
+    {
+#pragma unroll
+        for ( int i = 0; i < K; ++i )
+        {
+            wx[i] = sp[0].w[i];
+            gx[i] = sp[0].g[i];
+        }
+    }
+    else
+    {
+        wx[0] = sp[0].w[0];
+        gx[0] = sp[0].g[0];
+#pragma unroll
+        for ( int i = 1; i < K; ++i )
+        {
+            wx[0] = ( sx == i ) ? sp[0].w[i] : wx[0];
+            gx[0] = ( sx == i ) ? sp[0].g[i] : gx[0];
+        }
+    }
+    if constexpr ( S == K * K )
+    {
+        wy[0] = sp[1].w[0];
+        gy[0] = sp[1].g[0];
+#pragma unroll
+        for ( int j = 1; j < K; ++j )
+        {
+            wy[0] = ( sy == j ) ? sp[1].w[j] : wy[0];
+            gy[0] = ( sy == j ) ? sp[1].g[j] : gy[0];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for ( int j = 0; j < K; ++j )
+        {
+            wy[j] = sp[1].w[j];
+            gy[j] = sp[1].g[j];
+        }
+    }
+    // (j, k) products: A always, B and C for gradient-type forms.
+    Real A[NJ][K], B[NJ][K], C[NJ][K];
+#pragma unroll
+    for ( int j = 0; j < NJ; ++j )
+#pragma unroll
+        for ( int k = 0; k < K; ++k )
+        {
+            A[j][k] = wy[j] * sp[2].w[k];
+            if constexpr ( GRAD )
+            {
+                B[j][k] = gy[j] * sp[2].w[k];
+                C[j][k] = wy[j] * sp[2].g[k];
+            }
+        }
+#pragma unroll
+    for ( int i = 0; i < NI; ++i )
+    {
+        Real u[DG], v[DG], w[DG];
+#pragma unroll
+        for ( int d = 0; d < DG; ++d )
+            u[d] = v[d] = w[d] = Real( 0 );
+        static_for<NT>( [&]( auto I ) {
+            constexpr int t = I;
+            if constexpr ( TG::in( t, G ) )
+            {
+                constexpr int O = TG::offset( t, G );
+                slab_coeffs<FL::v[t]>( rp + RL::attr( t ), wx[i], gx[i], u + O, v + O, w + O );
+            }
+        } );
+#pragma unroll
+        for ( int j = 0; j < NJ; ++j )
+#pragma unroll
+            for ( int k = 0; k < K; ++k )
+                static_for<DG>( [&]( auto D ) {
+                    constexpr int d = D;
+                    Real& c = acc[( i * NJ + j ) * K + k][d];
+                    c = fma_r( A[j][k], u[d], c );
+                    if constexpr ( TG::grad_dof( G, d ) )
+                    {
+                        c = fma_r( B[j][k], v[d], c );
+                        c = fma_r( C[j][k], w[d], c );
+                    }
+                } );
+    }
+}
+
+// Task-table entry: base cell (x, y, z: 4 bits each), first record (15
+// bits), count (5).
+__device__ __forceinline__ unsigned pack_task( int lx, int ly, int lz, int start, int len )
+{
+    return ( (unsigned)lx << 28 ) | ( (unsigned)ly << 24 ) | ( (unsigned)lz << 20 ) |
+           ( (unsigned)start << 5 ) | (unsigned)len;
+}
+
+// The run table of a brick in two halves, so its global reads can be issued
+// early and committed later: lane r < nruns (<= 64) of wave 0 holds run r.
+struct RunLoad
+{
+    int64_t start;
+    int64_t count;
+};
+
+__device__ __forceinline__ void runs_issue( const TileArgs& ta, const MeshDev& m,
+                                            const TileGeom& g, RunLoad& rl )
+{
+    const int nj = g.c1[1] - g.c0[1];
+    const int nruns = ( g.c1[0] - g.c0[0] ) * nj;
+    const int r = threadIdx.x;
+    rl.start = 0;
+    rl.count = 0;
+    if ( r < nruns )
+    {
+        const int ci = g.c0[0] + r / nj, cj = g.c0[1] + r % nj;
+        const int64_t cfirst = ( (int64_t)ci * m.num_cell[1] + cj ) * m.num_cell[2] + g.c0[2];
+        const int64_t clast = cfirst + ( g.c1[2] - g.c0[2] ) - 1;
+        rl.start = ta.cell_offsets[cfirst];
+        rl.count = ta.cell_offsets[clast] + ta.cell_counts[clast] - rl.start;
+    }
+}
+
+// Wave 0 writes run_start and the inclusive prefix run_pre (visible after
+// the next barrier).
+__device__ __forceinline__ void runs_commit( const RunLoad& rl, const TileGeom& g,
+                                             int64_t* run_start, int* run_pre )
+{
+    if ( threadIdx.x >= 64 )
+        return;
+    const int nruns = ( g.c1[0] - g.c0[0] ) * ( g.c1[1] - g.c0[1] );
+    const int lane = threadIdx.x;
+    int v = (int)rl.count;
+#pragma unroll
+    for ( int o = 1; o < 64; o <<= 1 )
+    {
+        const int u = __shfl_up( v, o, 64 );
+        if ( lane >= o )
+            v += u;
+    }
+    if ( lane < nruns )
+    {
+        run_start[lane] = rl.start;
+        run_pre[lane + 1] = v;
+    }
+    if ( lane == 0 )
+        run_pre[0] = 0;
+}
+
+// Particles of one part held in registers between their loads and their use
+// (the loads of part k + 1 are in flight while part k accumulates).
+template <class Real, int MAXM, int NPT>
+struct PartPrefetch
+{
+    int p[MAXM];
+    double x[MAXM][3];
+    Real attr[MAXM][NPT];
+};
+
+template <class Real, bool CONTIG, int MAXM, int... FORMS>
+__device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileArgs& ta,
+                                               const int64_t* run_start, const int* run_pre,
+                                               int nruns, int pbeg, int nbatch,
+                                               PartPrefetch<Real, MAXM, RecordLayout<FORMS...>::len() - 3>& pf )
+{
+    using FL = FormList<FORMS...>;
+    using RL = RecordLayout<FORMS...>;
+#pragma unroll
+    for ( int mi = 0; mi < MAXM; ++mi )
+    {
+        const int i = (int)threadIdx.x + mi * BLOCK;
+        pf.p[mi] = -1;
+        if ( i >= nbatch )
+            continue;
+        const int64_t p = run_particle( ta, run_start, run_pre, nruns, pbeg + i );
+        pf.p[mi] = (int)p;
+        load_position<CONTIG>( a.points, p, pf.x[mi] );
+        static_for<FL::N>( [&]( auto I ) {
+            constexpr int t = I;
+            a.t[t].pin.template load<Real, np_of( FL::v[t] ), CONTIG>(
+                p, &pf.attr[mi][RL::attr( t ) - 3] );
+        } );
+    }
 }
 
 template <class Real, bool CONTIG, bool SHARED, int... FORMS>
-__global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
+__global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kernel(
     ScatterArgs a, TileArgs ta )
 {
     using FL = FormList<FORMS...>;
@@ -562,12 +859,12 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
     constexpr int DT = SD::total( SHARED );
     constexpr int RS = record_stride( RL::len(), (int)sizeof( Real ) );
     constexpr int MAXM = SCATTER_CAP_MAX / BLOCK;
-    constexpr int NGROUP = SHARED ? 1 : NT;
+    constexpr int NPT = RL::len() - 3;
     extern __shared__ __align__( 16 ) unsigned char smem[];
 
     // LDS layout (capacities fixed on the host for a full brick): FP64
     // window [node][DT], base-cell offsets, task counts, task table,
-    // records, run table, scan scratch.
+    // records, two run tables, scan scratch.
     double* win = reinterpret_cast<double*>( smem );
     size_t off = align16( sizeof( double ) * (size_t)DT * ta.max_win_nodes );
     int* cnt = reinterpret_cast<int*>( smem + off );
@@ -578,64 +875,108 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
     off = align16( off + sizeof( unsigned ) * ta.max_tasks );
     Real* rec = reinterpret_cast<Real*>( smem + off );
     off = align16( off + sizeof( Real ) * (size_t)RS * ta.cap );
-    int64_t* run_start = reinterpret_cast<int64_t*>( smem + off );
-    off = align16( off + sizeof( int64_t ) * ta.max_runs );
-    int* run_pre = reinterpret_cast<int*>( smem + off );
-    off = align16( off + sizeof( int ) * ( ta.max_runs + 1 ) );
+    int64_t* run_start[2];
+    int* run_pre[2];
+    for ( int b = 0; b < 2; ++b )
+    {
+        run_start[b] = reinterpret_cast<int64_t*>( smem + off );
+        off = align16( off + sizeof( int64_t ) * ta.max_runs );
+        run_pre[b] = reinterpret_cast<int*>( smem + off );
+        off = align16( off + sizeof( int ) * ( ta.max_runs + 1 ) );
+    }
     int* wsum = reinterpret_cast<int*>( smem + off );
 
     const Real rdx[3] = { (Real)a.mesh.rdx[0], (Real)a.mesh.rdx[1], (Real)a.mesh.rdx[2] };
     const int L = ta.sub_len;
+    const int W[3] = { ta.win[0], ta.win[1], ta.win[2] };
+    const int wnodes = W[0] * W[1] * W[2];
+    const float rW[3] = { 1.0f / (float)W[0], 1.0f / (float)W[1], 1.0f / (float)W[2] };
     const PartLoop pl( ta );
-    for ( int loc = pl.first; loc < pl.per_xcd; loc += pl.stride )
+    auto part_at = [&]( int l ) {
+        const int w = pl.xcd * pl.per_xcd + l;
+        return ( l < pl.per_xcd && w < pl.total ) ? w : -1;
+    };
+    int loc = pl.first;
+    int w = part_at( loc );
+    if ( w < 0 )
+        return;
+
+    // Prologue: the first part's run table and particle loads; zeroed
+    // window and counts.
+    TileGeom tg;
+    part_geometry( ta, a.mesh, w, tg );
     {
-        const int w = pl.xcd * pl.per_xcd + loc;
-        if ( w >= pl.total )
-            break;
-        TileGeom tg;
-        part_geometry( ta, a.mesh, w, tg );
+        RunLoad rl;
+        runs_issue( ta, a.mesh, tg, rl );
+        runs_commit( rl, tg, run_start[0], run_pre[0] );
+    }
+    for ( int e = threadIdx.x; e < DT * wnodes; e += BLOCK )
+        win[e] = 0.0;
+    for ( int c = threadIdx.x; c <= ta.max_base_cells; c += BLOCK )
+        cnt[c] = 0;
+    __syncthreads();
+    int buf = 0;
+    PartPrefetch<Real, MAXM, NPT> pf;
+    if ( CA_PART_AHEAD )
+    {
+        const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
+        const int total = run_pre[0][nruns];
+        const int pbeg = tg.part * ta.cap;
+        prefetch_part<Real, CONTIG, MAXM, FORMS...>( a, ta, run_start[0], run_pre[0], nruns,
+                                                     pbeg, min( ta.cap, total - pbeg ), pf );
+    }
+
+    for ( ;; )
+    {
+        // The next part: its run table is read now and committed after B.
+        const int loc1 = loc + pl.stride;
+        const int w1 = part_at( loc1 );
+        TileGeom tg1 = tg;
+        RunLoad rl1 = { 0, 0 };
+        if ( w1 >= 0 )
+        {
+            part_geometry( ta, a.mesh, w1, tg1 );
+            runs_issue( ta, a.mesh, tg1, rl1 );
+        }
         // Base cells of the brick and the node window they reach.
-        int blo[3], NB[3], W[3];
+        int blo[3], NB[3];
 #pragma unroll
         for ( int d = 0; d < 3; ++d )
         {
             blo[d] = tg.c0[d] + a.mesh.halo + BASE_SHIFT;
             NB[d] = ( tg.c1[d] - tg.c0[d] ) + BASE_SPREAD;
-            W[d] = ta.win[d];
         }
         const int nbt = NB[0] * NB[1] * NB[2];
-        const int wnodes = W[0] * W[1] * W[2];
-        for ( int e = threadIdx.x; e < DT * wnodes; e += BLOCK )
-            win[e] = 0.0;
-        for ( int c = threadIdx.x; c <= nbt; c += BLOCK )
-            cnt[c] = 0;
-        const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
-        const int total = setup_runs( ta, a.mesh, tg, run_start, run_pre );
-        const int pbeg = tg.part * ta.cap;
-        const int nbatch = min( ta.cap, total - pbeg );
+        if ( !CA_PART_AHEAD )
+        {
+            const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
+            const int total = run_pre[buf][nruns];
+            const int pbeg = tg.part * ta.cap;
+            prefetch_part<Real, CONTIG, MAXM, FORMS...>( a, ta, run_start[buf], run_pre[buf],
+                                                         nruns, pbeg, min( ta.cap, total - pbeg ),
+                                                         pf );
+        }
 
-        // A: stencil base cell of every particle of the part, counted per
-        // base cell (the slot is the particle's rank within its base cell);
-        // the particle's record (offsets, premultiplied attributes) is kept
-        // in registers until its place in the sorted order is known.
+        // A: stencil base cell of every particle of the part (from the
+        // prefetched registers), counted per base cell (the slot is the
+        // particle's rank within its base cell); the record (offsets,
+        // premultiplied attributes) stays in registers until its place in
+        // the sorted order is known.
         int key[MAXM];
         Real stash[MAXM][RL::len()];
 #pragma unroll
         for ( int mi = 0; mi < MAXM; ++mi )
         {
-            const int i = (int)threadIdx.x + mi * BLOCK;
             key[mi] = -1;
-            if ( i >= nbatch )
+            if ( pf.p[mi] < 0 )
                 continue;
-            const int64_t p = run_particle( ta, run_start, run_pre, nruns, pbeg + i );
-            double px[3];
-            load_position<CONTIG>( a.points, p, px );
+            const int64_t p = pf.p[mi];
             int bs[3];
             bool ok = true;
 #pragma unroll
             for ( int d = 0; d < 3; ++d )
                 ok = spline_frac<ORDER, Real>(
-                         logical_coord<ORDER>( px[d], a.mesh.low[d], a.mesh.rdx[d] ),
+                         logical_coord<ORDER>( pf.x[mi][d], a.mesh.low[d], a.mesh.rdx[d] ),
                          a.mesh.num_node[d], bs[d], stash[mi][d] ) &&
                      ok;
             if ( !ok )
@@ -659,18 +1000,16 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
                     constexpr int t = I;
                     constexpr int NP = np_of( FL::v[t] );
                     const Real m = (Real)a.t[t].mult;
-                    Real v[NP];
-                    a.t[t].pin.template load<Real, NP, CONTIG>( p, v );
 #pragma unroll
                     for ( int c = 0; c < NP; ++c )
-                        stash[mi][RL::attr( t ) + c] = v[c] * m;
+                        stash[mi][RL::attr( t ) + c] = pf.attr[mi][RL::attr( t ) - 3 + c] * m;
                 } );
             }
             else
             {
                 // Not sorted into this brick: add straight to the grid.
                 Spline3D<ORDER, Real, false> sp;
-                eval_spline( a.mesh, px, sp );
+                eval_spline( a.mesh, pf.x[mi], sp );
                 global_scatter_particle<Real, SHARED, FORMS...>( a, p, sp );
             }
         }
@@ -688,10 +1027,15 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
         for ( int b = threadIdx.x; b < nbt; b += BLOCK )
         {
             const int c0 = cnt[b], c = cnt[b + 1] - c0;
+            const int lx = b / ( NB[1] * NB[2] );
+            const int ly = ( b / NB[2] ) % NB[1];
+            const int lz = b % NB[2];
             int q = sub[b];
             for ( int r = 0; r < c; r += L )
-                task[q++] = pack_task( b, c0 + r, min( L, c - r ) );
+                task[q++] = pack_task( lx, ly, lz, c0 + r, min( L, c - r ) );
         }
+        if ( w1 >= 0 )
+            runs_commit( rl1, tg1, run_start[buf ^ 1], run_pre[buf ^ 1] );
         // C: records in base-cell order.
 #pragma unroll
         for ( int mi = 0; mi < MAXM; ++mi )
@@ -704,154 +1048,122 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
                 rp[c] = stash[mi][c];
         }
         __syncthreads();
-        // D: one lane per (run, group[, x-slab]): the run's particles
-        // accumulated over the stencil (or one x-slab of it) in registers,
-        // then added once into the window.  Each group's tasks start on a
-        // wave boundary, so a wave runs one group's code.
-        int seg[NGROUP + 1];
-        seg[0] = 0;
-        static_for<NGROUP>( [&]( auto GI ) {
-            constexpr int grp = GI;
-            constexpr int DG = SHARED ? 3 : ng_of( FL::v[grp] );
-            constexpr int S = ( DG > 1 ) ? K : 1;
-            seg[grp + 1] = seg[grp] + ( ntask * S + 63 ) / 64 * 64;
-        } );
-        for ( int tt = threadIdx.x; tt < seg[NGROUP]; tt += BLOCK )
+        // The next part's particle reads are issued here and complete while
+        // this part accumulates.
+        if ( CA_PART_AHEAD && w1 >= 0 )
         {
-            static_for<NGROUP>( [&]( auto GI ) {
+            const int nruns1 = ( tg1.c1[0] - tg1.c0[0] ) * ( tg1.c1[1] - tg1.c0[1] );
+            const int total1 = run_pre[buf ^ 1][nruns1];
+            const int pbeg1 = tg1.part * ta.cap;
+            prefetch_part<Real, CONTIG, MAXM, FORMS...>( a, ta, run_start[buf ^ 1],
+                                                         run_pre[buf ^ 1], nruns1, pbeg1,
+                                                         min( ta.cap, total1 - pbeg1 ), pf );
+        }
+        // D: one lane per (task, term group, stencil slab).  The lane walks
+        // its task's records, accumulates its group's node values over its
+        // slab in registers, and adds them once into the window.  Each
+        // group's lanes start on a wave boundary, so a wave runs one group's
+        // code; the lanes of one task read the same records (broadcast).
+        {
+            using TGR = TermGroups<SHARED, FORMS...>;
+            constexpr int NGR = TGR::N;
+            constexpr int RLEN = RL::len();
+            int seg[NGR + 1];
+            seg[0] = 0;
+            static_for<NGR>( [&]( auto GI ) {
                 constexpr int grp = GI;
-                constexpr int DG = SHARED ? 3 : ng_of( FL::v[grp] );
-                constexpr int O = SHARED ? 0 : SD::offset( grp, false );
-                constexpr bool SLABS = DG > 1;
-                constexpr int S = SLABS ? K : 1;
-                constexpr int NI = SLABS ? 1 : K;
-                // Gradient-type forms in the group need the B, C products.
-                constexpr bool GRAD = [] {
-                    for ( int t = 0; t < NT; ++t )
-                        if ( ( SHARED || t == grp ) && !form_value_type( FL::v[t] ) )
-                            return true;
-                    return false;
-                }();
-                const int local = tt - seg[grp];
-                if ( local < 0 || local >= ntask * S )
-                    return;
-                const int q = SLABS ? local / K : local;
-                const int sa = SLABS ? local - q * K : 0;
-                const unsigned e = task[q];
-                const int b = (int)( e >> 20 );
-                const int r0 = (int)( ( e >> 5 ) & 0x7fff );
-                const int r1 = r0 + (int)( e & 31 );
-                Real acc[NI * K * K][DG];
+                constexpr int S = SlabPlan<TGR::dofs( grp )>::S;
+                seg[grp + 1] = seg[grp] + ( ntask * S + 63 ) / 64 * 64;
+            } );
+            for ( int tt = threadIdx.x; tt < seg[NGR]; tt += BLOCK )
+            {
+                static_for<NGR>( [&]( auto GI ) {
+                    constexpr int grp = GI;
+                    constexpr int DG = TGR::dofs( grp );
+                    using SP = SlabPlan<DG>;
+                    constexpr int S = SP::S;
+                    constexpr int NI = SP::NI;
+                    constexpr int NJ = SP::NJ;
+                    constexpr int NN = NI * NJ * K;
+                    const int local = tt - seg[grp];
+                    if ( local < 0 || local >= ntask * S )
+                        return;
+                    const int q = ( S == 1 ) ? local : local / S;
+                    const int sa = ( S == 1 ) ? 0 : local - q * S;
+                    const int sx = ( S == 1 ) ? 0 : ( S == K ? sa : sa / K );
+                    const int sy = ( S == K * K ) ? sa - sx * K : 0;
+                    const unsigned e = task[q];
+                    const int lx = (int)( e >> 28 );
+                    const int ly = (int)( ( e >> 24 ) & 15 );
+                    const int lz = (int)( ( e >> 20 ) & 15 );
+                    const int r0 = (int)( ( e >> 5 ) & 0x7fff );
+                    const int r1 = r0 + (int)( e & 31 );
+                    Real acc[NN][DG];
 #pragma unroll
-                for ( int n = 0; n < NI * K * K; ++n )
-#pragma unroll
-                    for ( int d = 0; d < DG; ++d )
-                        acc[n][d] = Real( 0 );
-                for ( int r = r0; r < r1; ++r )
-                {
-                    const Real* rp = rec + (size_t)RS * r;
-                    Spline1D<ORDER, Real> sp[3];
-#pragma unroll
-                    for ( int d = 0; d < 3; ++d )
-                        spline_weights<ORDER, Real, false>( rp[d], rdx[d], sp[d] );
-                    Real wx[NI], gx[NI];
-                    if constexpr ( SLABS )
-                    {
-                        wx[0] = sp[0].w[0];
-                        gx[0] = sp[0].g[0];
-#pragma unroll
-                        for ( int ii = 1; ii < K; ++ii )
-                        {
-                            wx[0] = ( sa == ii ) ? sp[0].w[ii] : wx[0];
-                            gx[0] = ( sa == ii ) ? sp[0].g[ii] : gx[0];
-                        }
-                    }
-                    else
-                    {
-#pragma unroll
-                        for ( int i = 0; i < NI; ++i )
-                        {
-                            wx[i] = sp[0].w[i];
-                            gx[i] = sp[0].g[i];
-                        }
-                    }
-                    Real Pm[NT][9];
-                    static_for<NT>( [&]( auto I ) {
-                        constexpr int t = I;
-                        if constexpr ( SHARED || t == grp )
-                        {
-#pragma unroll
-                            for ( int c = 0; c < np_of( FL::v[t] ); ++c )
-                                Pm[t][c] = rp[RL::attr( t ) + c];
-                        }
-                    } );
-                    // (j, k) products of the slab: A always, B and C for
-                    // gradient-type forms.
-                    Real A[K][K], B[K][K], C[K][K];
-#pragma unroll
-                    for ( int j = 0; j < K; ++j )
-#pragma unroll
-                        for ( int k = 0; k < K; ++k )
-                        {
-                            A[j][k] = sp[1].w[j] * sp[2].w[k];
-                            if constexpr ( GRAD )
-                            {
-                                B[j][k] = sp[1].g[j] * sp[2].w[k];
-                                C[j][k] = sp[1].w[j] * sp[2].g[k];
-                            }
-                        }
-#pragma unroll
-                    for ( int i = 0; i < NI; ++i )
-                    {
-                        Real u[DG], v[DG], w[DG];
+                    for ( int n = 0; n < NN; ++n )
 #pragma unroll
                         for ( int d = 0; d < DG; ++d )
-                            u[d] = v[d] = w[d] = Real( 0 );
-                        static_for<NT>( [&]( auto I ) {
-                            constexpr int t = I;
-                            if constexpr ( SHARED || t == grp )
-                                slab_coeffs<FL::v[t]>( Pm[t], wx[i], gx[i], u, v, w );
-                        } );
+                            acc[n][d] = Real( 0 );
+#if CA_REC_AHEAD
+                    Real cur[RLEN];
 #pragma unroll
-                        for ( int j = 0; j < K; ++j )
+                    for ( int c = 0; c < RLEN; ++c )
+                        cur[c] = rec[(size_t)RS * r0 + c];
+                    for ( int r = r0; r < r1; ++r )
+                    {
+                        const int rn = min( r + 1, r1 - 1 );
+                        Real nxt[RLEN];
+#pragma unroll
+                        for ( int c = 0; c < RLEN; ++c )
+                            nxt[c] = rec[(size_t)RS * rn + c];
+                        scatter_record<Real, SHARED, grp, S, FORMS...>( cur, rdx, sx, sy, acc );
+#pragma unroll
+                        for ( int c = 0; c < RLEN; ++c )
+                            cur[c] = nxt[c];
+                    }
+#else
+                    for ( int r = r0; r < r1; ++r )
+                    {
+                        Real cur[RLEN];
+#pragma unroll
+                        for ( int c = 0; c < RLEN; ++c )
+                            cur[c] = rec[(size_t)RS * r + c];
+                        scatter_record<Real, SHARED, grp, S, FORMS...>( cur, rdx, sx, sy, acc );
+                    }
+#endif
+                    double* wb = win + ( ( lx + sx ) * W[1] + ly + sy ) * W[2] * DT + lz * DT;
+#pragma unroll
+                    for ( int i = 0; i < NI; ++i )
+#pragma unroll
+                        for ( int j = 0; j < NJ; ++j )
 #pragma unroll
                             for ( int k = 0; k < K; ++k )
-#pragma unroll
-                                for ( int d = 0; d < DG; ++d )
-                                {
-                                    Real& c = acc[( i * K + j ) * K + k][d];
-                                    c = fma_r( A[j][k], u[d], c );
-                                    if constexpr ( GRAD )
+                            {
+                                double* q2 = wb + ( ( i * W[1] + j ) * W[2] + k ) * DT;
+                                static_for<NT>( [&]( auto I ) {
+                                    constexpr int t = I;
+                                    if constexpr ( TGR::in( t, grp ) &&
+                                                   ( !SHARED || t == 0 ) )
                                     {
-                                        c = fma_r( B[j][k], v[d], c );
-                                        c = fma_r( C[j][k], w[d], c );
+                                        constexpr int NGT = SHARED ? 3 : ng_of( FL::v[t] );
+                                        constexpr int OW = SD::offset( t, SHARED );
+                                        constexpr int OA = TGR::offset( t, grp );
+#pragma unroll
+                                        for ( int d = 0; d < NGT; ++d )
+                                            atomicAdd( q2 + OW + d,
+                                                       (double)acc[( i * NJ + j ) * K + k][OA + d] );
                                     }
-                                }
-                    }
-                }
-                const int lx = b / ( NB[1] * NB[2] );
-                const int ly = ( b / NB[2] ) % NB[1];
-                const int lz = b % NB[2];
-#pragma unroll
-                for ( int i = 0; i < NI; ++i )
-#pragma unroll
-                    for ( int j = 0; j < K; ++j )
-#pragma unroll
-                        for ( int k = 0; k < K; ++k )
-                        {
-                            const int node =
-                                ( ( lx + sa + i ) * W[1] + ly + j ) * W[2] + lz + k;
-#pragma unroll
-                            for ( int d = 0; d < DG; ++d )
-                                atomicAdd( win + node * DT + O + d,
-                                           (double)acc[( i * K + j ) * K + k][d] );
-                        }
-            } );
+                                } );
+                            }
+                } );
+            }
         }
         __syncthreads();
 
         // E: flush the window, one global atomic per non-zero node value,
-        // walking the window with the grid's unit-stride dimension fastest.
+        // walking the window with the grid's unit-stride dimension fastest;
+        // every entry is zeroed by the thread that flushed it, and the
+        // counts are cleared, for the next part.
         static_for<( SHARED ? 1 : NT )>( [&]( auto I ) {
             constexpr int t = I;
             constexpr int NG = SHARED ? 3 : ng_of( FL::v[t] );
@@ -860,9 +1172,10 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
             for ( int f = threadIdx.x; f < wnodes * NG; f += BLOCK )
             {
                 int q[3], d;
-                window_coords( ta, f, NG, W, q, d );
-                const Real v =
-                    (Real)win[( ( q[0] * W[1] + q[1] ) * W[2] + q[2] ) * DT + O + d];
+                window_coords<NG>( ta, f, W, rW, q, d );
+                double* src = win + ( ( q[0] * W[1] + q[1] ) * W[2] + q[2] ) * DT + O + d;
+                const Real v = (Real)*src;
+                *src = 0.0;
                 if ( v == Real( 0 ) )
                     continue;
                 const int gi = blo[0] + q[0], gj = blo[1] + q[1], gk = blo[2] + q[2];
@@ -874,8 +1187,16 @@ __global__ __launch_bounds__( BLOCK ) void scatter_cells_kernel(
                             v );
             }
         } );
-        // The next part reuses the window and the tables.
+        for ( int c = threadIdx.x; c <= ta.max_base_cells; c += BLOCK )
+            cnt[c] = 0;
+        // The next part reuses the window, the counts and the tables.
         __syncthreads();
+        if ( w1 < 0 )
+            break;
+        loc = loc1;
+        w = w1;
+        tg = tg1;
+        buf ^= 1;
     }
 }
 

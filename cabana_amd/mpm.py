@@ -30,13 +30,33 @@ class PeriodicBox:
         self.cell_size = cell_size
 
 
+class _Permute(torch.autograd.Function):
+    """out[i] = t[perm[i]] (Cabana_Sort.hpp:600-656 applied to a tensor).
+    perm is a bijection, so the adjoint is the scatter grad_t[perm[i]] =
+    grad_out[i], with no accumulation."""
+
+    @staticmethod
+    def forward(ctx, t, perm):
+        ctx.save_for_backward(perm)
+        return t.index_select(0, perm)
+
+    @staticmethod
+    def backward(ctx, g):
+        (perm,) = ctx.saved_tensors
+        return torch.empty_like(g).index_copy_(0, perm, g.contiguous()), None
+
+
+def permute_tensor(t, perm):
+    return _Permute.apply(t, perm)
+
+
 def sort_by_cell(box, x, *fields):
     """Cell-sorts positions and fields; returns (binning, x, fields...).  The
     binning is built from detached positions; the permutation itself is a
     gather, so gradients flow back to the unsorted tensors."""
     lcl = createLinkedCellList(x.detach(), [box.cell_size] * 3, [0.0] * 3, [box.length] * 3)
     perm = lcl.permutes
-    return (lcl, x[perm]) + tuple(f[perm] for f in fields)
+    return (lcl, permute_tensor(x, perm)) + tuple(permute_tensor(f, perm) for f in fields)
 
 
 def explicit_step(box, x, v, m, dt=0.1, body_force=(0.0, 0.0, -1e-3), resort=True):

@@ -67,6 +67,26 @@ def test_linked_cell_bit_exact(ca, oracle, n, ppc):
     assert np.array_equal(lcl.permutes.cpu().numpy(), perm)
 
 
+@pytest.mark.parametrize("case", ["three_passes", "crowded_bin", "subrange"])
+def test_linked_cell_sort_shapes(ca, oracle, case):
+    """Bin counts that need three radix passes (2^18 bins), one bin holding
+    most particles, and a begin/end subrange: bit-exact Serial build."""
+    rng = np.random.default_rng({"three_passes": 1, "crowded_bin": 2, "subrange": 3}[case])
+    nx = 64
+    lo, hi = np.zeros(3), np.full(3, float(nx))
+    n = 300001
+    pos = rng.uniform(0, nx, size=(n, 3))
+    if case == "crowded_bin":
+        pos[: 2 * n // 3] = 17.0 + rng.uniform(0, 1, size=(2 * n // 3, 3))
+    begin, end = (0, n) if case != "subrange" else (12345, n - 777)
+    counts, offsets, perm, _ = oracle.lcl_build(pos, [1.0] * 3, lo, hi, begin=begin, end=end)
+    lcl = ca.createLinkedCellList(torch.tensor(pos, device="cuda"), [1.0] * 3, lo, hi,
+                                  begin, end)
+    assert np.array_equal(lcl.counts.cpu().numpy(), counts)
+    assert np.array_equal(lcl.offsets.cpu().numpy(), offsets)
+    assert np.array_equal(lcl.permutes.cpu().numpy(), perm)
+
+
 def test_linked_cell_aosoa_positions_and_permute(ca, oracle):
     """Positions and a tensor member in one AoSoA (VL = 64): build from the
     slice, permute the slice, compare bitwise with the oracle."""
