@@ -517,7 +517,7 @@ size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng, int nslot,
         b = a16( b + 8 * runs );
         b = a16( b + 4 * ( runs + 1 ) );
     }
-    b += 4 * ( TILE_BLOCK / 64 ) + 16;
+    b += 4 * 16 + 16; // per-wave scan sums of a workgroup of up to 1024 threads
     return b;
 }
 

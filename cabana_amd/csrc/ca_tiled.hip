@@ -380,11 +380,12 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
 // Cell-register scatter
 // ---------------------------------------------------------------------------
 
-// Exclusive scan of n (<= BLOCK * 16) ints in LDS, in place; returns the sum
-// to every thread.
+// Exclusive scan of n (<= NTH * 16) ints in LDS, in place, by a workgroup of
+// NTH threads; returns the sum to every thread.
+template <int NTH>
 __device__ __forceinline__ int block_exclusive_scan( int* v, int n, int* wsum )
 {
-    const int per = ( n + BLOCK - 1 ) / BLOCK;
+    const int per = ( n + NTH - 1 ) / NTH;
     const int lo = threadIdx.x * per;
     const int hi = min( lo + per, n );
     int s = 0;
@@ -402,10 +403,14 @@ __device__ __forceinline__ int block_exclusive_scan( int* v, int n, int* wsum )
     if ( lane == 63 )
         wsum[wv] = x;
     __syncthreads();
-    int wbase = 0;
-    for ( int w = 0; w < wv; ++w )
-        wbase += wsum[w];
-    const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    int wbase = 0, total = 0;
+#pragma unroll
+    for ( int w = 0; w < NTH / 64; ++w )
+    {
+        const int c = wsum[w];
+        wbase += w < wv ? c : 0;
+        total += c;
+    }
     int run = wbase + x - s;
     for ( int i = lo; i < hi; ++i )
     {
@@ -568,9 +573,16 @@ struct RecordLayout
 #ifndef CA_PART_AHEAD
 #define CA_PART_AHEAD 1
 #endif
+#ifndef CA_SCATTER_BLOCK
+#define CA_SCATTER_BLOCK 256
+#endif
 #ifndef CA_SCATTER_WAVES
 #define CA_SCATTER_WAVES 2
 #endif
+// Workgroup size of the scatter and its minimum waves per SIMD (the
+// register budget): two 256-thread workgroups per CU, so one workgroup's
+// memory phases overlap the other's accumulation.
+constexpr int SBLOCK = CA_SCATTER_BLOCK;
 template <int DT>
 struct SlabPlan
 {
@@ -706,20 +718,6 @@ __device__ __forceinline__ void scatter_record(
             gy[j] = sp[1].g[j];
         }
     }
-    // (j, k) products: A always, B and C for gradient-type forms.
-    Real A[NJ][K], B[NJ][K], C[NJ][K];
-#pragma unroll
-    for ( int j = 0; j < NJ; ++j )
-#pragma unroll
-        for ( int k = 0; k < K; ++k )
-        {
-            A[j][k] = wy[j] * sp[2].w[k];
-            if constexpr ( GRAD )
-            {
-                B[j][k] = gy[j] * sp[2].w[k];
-                C[j][k] = wy[j] * sp[2].g[k];
-            }
-        }
 #pragma unroll
     for ( int i = 0; i < NI; ++i )
     {
@@ -735,20 +733,27 @@ __device__ __forceinline__ void scatter_record(
                 slab_coeffs<FL::v[t]>( rp + RL::attr( t ), wx[i], gx[i], u + O, v + O, w + O );
             }
         } );
+        // (j, k) products, formed where they are used: A always, B and C
+        // for gradient-type forms.
 #pragma unroll
         for ( int j = 0; j < NJ; ++j )
 #pragma unroll
             for ( int k = 0; k < K; ++k )
+            {
+                const Real A = wy[j] * sp[2].w[k];
+                const Real B = GRAD ? gy[j] * sp[2].w[k] : Real( 0 );
+                const Real C = GRAD ? wy[j] * sp[2].g[k] : Real( 0 );
                 static_for<DG>( [&]( auto D ) {
                     constexpr int d = D;
                     Real& c = acc[( i * NJ + j ) * K + k][d];
-                    c = fma_r( A[j][k], u[d], c );
+                    c = fma_r( A, u[d], c );
                     if constexpr ( TG::grad_dof( G, d ) )
                     {
-                        c = fma_r( B[j][k], v[d], c );
-                        c = fma_r( C[j][k], w[d], c );
+                        c = fma_r( B, v[d], c );
+                        c = fma_r( C, w[d], c );
                     }
                 } );
+            }
     }
 }
 
@@ -822,7 +827,7 @@ struct PartPrefetch
     Real attr[MAXM][NPT];
 };
 
-template <class Real, bool CONTIG, int MAXM, int... FORMS>
+template <class Real, bool CONTIG, int MAXM, int NTH, int... FORMS>
 __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileArgs& ta,
                                                const int64_t* run_start, const int* run_pre,
                                                int nruns, int pbeg, int nbatch,
@@ -833,7 +838,7 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
 #pragma unroll
     for ( int mi = 0; mi < MAXM; ++mi )
     {
-        const int i = (int)threadIdx.x + mi * BLOCK;
+        const int i = (int)threadIdx.x + mi * NTH;
         pf.p[mi] = -1;
         if ( i >= nbatch )
             continue;
@@ -849,7 +854,7 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
 }
 
 template <class Real, bool CONTIG, bool SHARED, int... FORMS>
-__global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kernel(
+__global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kernel(
     ScatterArgs a, TileArgs ta )
 {
     using FL = FormList<FORMS...>;
@@ -858,7 +863,7 @@ __global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kerne
     constexpr int NT = FL::N;
     constexpr int DT = SD::total( SHARED );
     constexpr int RS = record_stride( RL::len(), (int)sizeof( Real ) );
-    constexpr int MAXM = SCATTER_CAP_MAX / BLOCK;
+    constexpr int MAXM = ( SCATTER_CAP_MAX + SBLOCK - 1 ) / SBLOCK;
     constexpr int NPT = RL::len() - 3;
     extern __shared__ __align__( 16 ) unsigned char smem[];
 
@@ -910,9 +915,9 @@ __global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kerne
         runs_issue( ta, a.mesh, tg, rl );
         runs_commit( rl, tg, run_start[0], run_pre[0] );
     }
-    for ( int e = threadIdx.x; e < DT * wnodes; e += BLOCK )
+    for ( int e = threadIdx.x; e < DT * wnodes; e += SBLOCK )
         win[e] = 0.0;
-    for ( int c = threadIdx.x; c <= ta.max_base_cells; c += BLOCK )
+    for ( int c = threadIdx.x; c <= ta.max_base_cells; c += SBLOCK )
         cnt[c] = 0;
     __syncthreads();
     int buf = 0;
@@ -922,7 +927,7 @@ __global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kerne
         const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
         const int total = run_pre[0][nruns];
         const int pbeg = tg.part * ta.cap;
-        prefetch_part<Real, CONTIG, MAXM, FORMS...>( a, ta, run_start[0], run_pre[0], nruns,
+        prefetch_part<Real, CONTIG, MAXM, SBLOCK, FORMS...>( a, ta, run_start[0], run_pre[0], nruns,
                                                      pbeg, min( ta.cap, total - pbeg ), pf );
     }
 
@@ -952,7 +957,7 @@ __global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kerne
             const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
             const int total = run_pre[buf][nruns];
             const int pbeg = tg.part * ta.cap;
-            prefetch_part<Real, CONTIG, MAXM, FORMS...>( a, ta, run_start[buf], run_pre[buf],
+            prefetch_part<Real, CONTIG, MAXM, SBLOCK, FORMS...>( a, ta, run_start[buf], run_pre[buf],
                                                          nruns, pbeg, min( ta.cap, total - pbeg ),
                                                          pf );
         }
@@ -1016,15 +1021,15 @@ __global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kerne
         __syncthreads();
         // B: base-cell offsets (cnt[nbt] = sorted particles), then the task
         // table: every base cell's records cut into runs of at most L.
-        const int nsorted = block_exclusive_scan( cnt, nbt, wsum );
+        const int nsorted = block_exclusive_scan<SBLOCK>( cnt, nbt, wsum );
         if ( threadIdx.x == 0 )
             cnt[nbt] = nsorted;
         __syncthreads();
-        for ( int b = threadIdx.x; b < nbt; b += BLOCK )
+        for ( int b = threadIdx.x; b < nbt; b += SBLOCK )
             sub[b] = ( cnt[b + 1] - cnt[b] + L - 1 ) / L;
         __syncthreads();
-        const int ntask = block_exclusive_scan( sub, nbt, wsum );
-        for ( int b = threadIdx.x; b < nbt; b += BLOCK )
+        const int ntask = block_exclusive_scan<SBLOCK>( sub, nbt, wsum );
+        for ( int b = threadIdx.x; b < nbt; b += SBLOCK )
         {
             const int c0 = cnt[b], c = cnt[b + 1] - c0;
             const int lx = b / ( NB[1] * NB[2] );
@@ -1055,7 +1060,7 @@ __global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kerne
             const int nruns1 = ( tg1.c1[0] - tg1.c0[0] ) * ( tg1.c1[1] - tg1.c0[1] );
             const int total1 = run_pre[buf ^ 1][nruns1];
             const int pbeg1 = tg1.part * ta.cap;
-            prefetch_part<Real, CONTIG, MAXM, FORMS...>( a, ta, run_start[buf ^ 1],
+            prefetch_part<Real, CONTIG, MAXM, SBLOCK, FORMS...>( a, ta, run_start[buf ^ 1],
                                                          run_pre[buf ^ 1], nruns1, pbeg1,
                                                          min( ta.cap, total1 - pbeg1 ), pf );
         }
@@ -1075,7 +1080,7 @@ __global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kerne
                 constexpr int S = SlabPlan<TGR::dofs( grp )>::S;
                 seg[grp + 1] = seg[grp] + ( ntask * S + 63 ) / 64 * 64;
             } );
-            for ( int tt = threadIdx.x; tt < seg[NGR]; tt += BLOCK )
+            for ( int tt = threadIdx.x; tt < seg[NGR]; tt += SBLOCK )
             {
                 static_for<NGR>( [&]( auto GI ) {
                     constexpr int grp = GI;
@@ -1169,7 +1174,7 @@ __global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kerne
             constexpr int NG = SHARED ? 3 : ng_of( FL::v[t] );
             constexpr int O = SD::offset( t, SHARED );
             const GFieldDev& g = a.t[t].grid;
-            for ( int f = threadIdx.x; f < wnodes * NG; f += BLOCK )
+            for ( int f = threadIdx.x; f < wnodes * NG; f += SBLOCK )
             {
                 int q[3], d;
                 window_coords<NG>( ta, f, W, rW, q, d );
@@ -1187,7 +1192,7 @@ __global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kerne
                             v );
             }
         } );
-        for ( int c = threadIdx.x; c <= ta.max_base_cells; c += BLOCK )
+        for ( int c = threadIdx.x; c <= ta.max_base_cells; c += SBLOCK )
             cnt[c] = 0;
         // The next part reuses the window, the counts and the tables.
         __syncthreads();
@@ -1206,7 +1211,7 @@ __global__ __launch_bounds__( BLOCK, CA_SCATTER_WAVES ) void scatter_cells_kerne
 
 template <class KernelPtr, class Args>
 hipError_t launch_tile_kernel( KernelPtr k, const Args& a, const TileArgs& t,
-                               size_t lds, hipStream_t s )
+                               size_t lds, hipStream_t s, int block = BLOCK )
 {
     if ( lds > 64 * 1024 )
     {
@@ -1216,7 +1221,7 @@ hipError_t launch_tile_kernel( KernelPtr k, const Args& a, const TileArgs& t,
         if ( e != hipSuccess )
             return e;
     }
-    hipLaunchKernelGGL( k, dim3( (unsigned)t.grid ), dim3( BLOCK ), lds, s, a, t );
+    hipLaunchKernelGGL( k, dim3( (unsigned)t.grid ), dim3( block ), lds, s, a, t );
     return hipGetLastError();
 }
 
@@ -1288,7 +1293,7 @@ hipError_t scatter_cells_launch( int plan, const ScatterArgs& a, const TileArgs&
         {
 #define CA_CASE( F )                                                           \
     case F:                                                                    \
-        return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, false, F>, a, t, lds, s );
+        return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, false, F>, a, t, lds, s, SBLOCK );
             CA_CASE( FORM_SV )
             CA_CASE( FORM_VV )
             CA_CASE( FORM_SG )
@@ -1302,10 +1307,11 @@ hipError_t scatter_cells_launch( int plan, const ScatterArgs& a, const TileArgs&
     }
     if ( plan == PLAN_MPM_P2G )
         return launch_tile_kernel(
-            scatter_cells_kernel<Real, CONTIG, false, FORM_SV, FORM_VV, FORM_TD>, a, t, lds, s );
+            scatter_cells_kernel<Real, CONTIG, false, FORM_SV, FORM_VV, FORM_TD>, a, t, lds, s,
+            SBLOCK );
     if ( plan == PLAN_MPM_G2P )
         return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, true, FORM_VV, FORM_TDT>,
-                                   a, t, lds, s );
+                                   a, t, lds, s, SBLOCK );
     return hipErrorInvalidValue;
 }
 
