@@ -2,6 +2,7 @@
 #include "ca_internal.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -470,6 +471,18 @@ enum TileKind
 };
 
 constexpr int TILE_BLOCK = 256;
+
+// Workgroups of a tiled launch (a multiple of 8, one range per XCD): 4096 by
+// default; CABANA_AMD_MAX_TILE_GROUPS lowers it (tests use it to give every
+// workgroup many parts).
+int64_t max_tile_groups()
+{
+    const char* e = std::getenv( "CABANA_AMD_MAX_TILE_GROUPS" );
+    long x = e ? std::strtol( e, nullptr, 10 ) : 0;
+    if ( x < 8 || x > 4096 )
+        x = 4096;
+    return (int64_t)( x / 8 * 8 );
+}
 constexpr int SCATTER_SUB_LEN = 12;
 
 struct TileShape
@@ -509,7 +522,6 @@ size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng, int nslot,
     const size_t rs = (size_t)record_stride( rec_len, (int)real );
     b = a16( 8 * (size_t)dt * wn );
     b = a16( b + 4 * ( nbase + 1 ) );
-    b = a16( b + 4 * ( nbase + 1 ) );
     b = a16( b + 4 * (size_t)sh.max_tasks );
     b = a16( b + real * rs * (size_t)sh.cap );
     for ( int buf = 0; buf < 2; ++buf )
@@ -517,7 +529,8 @@ size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng, int nslot,
         b = a16( b + 8 * runs );
         b = a16( b + 4 * ( runs + 1 ) );
     }
-    b += 4 * 16 + 16; // per-wave scan sums of a workgroup of up to 1024 threads
+    b = a16( b + 8 * 128 ); // part-table entries
+    b += 4 * 16 + 16;       // per-wave scan sums of a workgroup of up to 1024 threads
     return b;
 }
 
@@ -687,7 +700,7 @@ cabana_amd_status make_parts( cabana_amd_context* ctx, const MeshDev& m,
     t.parts = parts;
     t.num_parts = num_parts;
     int64_t grid = ( max_parts + 7 ) / 8 * 8;
-    t.grid = (int32_t)( grid < 4096 ? grid : 4096 );
+    t.grid = (int32_t)( grid < max_tile_groups() ? grid : max_tile_groups() );
     return CABANA_AMD_OK;
 }
 

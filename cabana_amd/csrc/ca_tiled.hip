@@ -105,10 +105,9 @@ struct PartLoop
     }
 };
 
-__device__ __forceinline__ void part_geometry( const TileArgs& ta, const MeshDev& m,
-                                               int w, TileGeom& g )
+__device__ __forceinline__ void part_geometry_of( const TileArgs& ta, const MeshDev& m,
+                                                  int2 pr, TileGeom& g )
 {
-    const int2 pr = ta.parts[w];
     g.brick = pr.x;
     g.part = pr.y;
     const int ti = g.brick / ( ta.ntile[1] * ta.ntile[2] );
@@ -120,6 +119,12 @@ __device__ __forceinline__ void part_geometry( const TileArgs& ta, const MeshDev
 #pragma unroll
     for ( int d = 0; d < 3; ++d )
         g.c1[d] = min( g.c0[d] + ta.tile[d], m.num_cell[d] );
+}
+
+__device__ __forceinline__ void part_geometry( const TileArgs& ta, const MeshDev& m,
+                                               int w, TileGeom& g )
+{
+    part_geometry_of( ta, m, ta.parts[w], g );
 }
 
 // Fills the run table of the brick (one run per (i, j) cell row) and returns
@@ -382,15 +387,24 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
 
 // Exclusive scan of n (<= NTH * 16) ints in LDS, in place, by a workgroup of
 // NTH threads; returns the sum to every thread.
-template <int NTH>
-__device__ __forceinline__ int block_exclusive_scan( int* v, int n, int* wsum )
+struct ScanIdentity
+{
+    __device__ int operator()( int x ) const { return x; }
+};
+
+// Exclusive scan of f(v[i]) (f applied as the values are read).
+template <int NTH, class F = ScanIdentity>
+__device__ __forceinline__ int block_exclusive_scan( int* v, int n, int* wsum, F f = F() )
 {
     const int per = ( n + NTH - 1 ) / NTH;
     const int lo = threadIdx.x * per;
     const int hi = min( lo + per, n );
     int s = 0;
     for ( int i = lo; i < hi; ++i )
+    {
+        v[i] = f( v[i] );
         s += v[i];
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int x = s;
 #pragma unroll
@@ -835,6 +849,19 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
 {
     using FL = FormList<FORMS...>;
     using RL = RecordLayout<FORMS...>;
+    // Run of every slot: count the run starts at or below its flattened
+    // index (broadcast LDS reads, no dependent search).
+    int run[MAXM];
+#pragma unroll
+    for ( int mi = 0; mi < MAXM; ++mi )
+        run[mi] = 0;
+    for ( int q = 1; q < nruns; ++q )
+    {
+        const int rp = run_pre[q];
+#pragma unroll
+        for ( int mi = 0; mi < MAXM; ++mi )
+            run[mi] += ( rp <= pbeg + (int)threadIdx.x + mi * NTH ) ? 1 : 0;
+    }
 #pragma unroll
     for ( int mi = 0; mi < MAXM; ++mi )
     {
@@ -842,7 +869,7 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
         pf.p[mi] = -1;
         if ( i >= nbatch )
             continue;
-        const int64_t p = run_particle( ta, run_start, run_pre, nruns, pbeg + i );
+        const int64_t p = ta.bin_begin + run_start[run[mi]] + ( pbeg + i - run_pre[run[mi]] );
         pf.p[mi] = (int)p;
         load_position<CONTIG>( a.points, p, pf.x[mi] );
         static_for<FL::N>( [&]( auto I ) {
@@ -868,13 +895,11 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
     extern __shared__ __align__( 16 ) unsigned char smem[];
 
     // LDS layout (capacities fixed on the host for a full brick): FP64
-    // window [node][DT], base-cell offsets, task counts, task table,
-    // records, two run tables, scan scratch.
+    // window [node][DT], base-cell offsets, task table, records, two run
+    // tables, part-table entries, scan scratch.
     double* win = reinterpret_cast<double*>( smem );
     size_t off = align16( sizeof( double ) * (size_t)DT * ta.max_win_nodes );
     int* cnt = reinterpret_cast<int*>( smem + off );
-    off = align16( off + sizeof( int ) * ( ta.max_base_cells + 1 ) );
-    int* sub = reinterpret_cast<int*>( smem + off );
     off = align16( off + sizeof( int ) * ( ta.max_base_cells + 1 ) );
     unsigned* task = reinterpret_cast<unsigned*>( smem + off );
     off = align16( off + sizeof( unsigned ) * ta.max_tasks );
@@ -889,6 +914,8 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
         run_pre[b] = reinterpret_cast<int*>( smem + off );
         off = align16( off + sizeof( int ) * ( ta.max_runs + 1 ) );
     }
+    int2* ptab = reinterpret_cast<int2*>( smem + off );
+    off = align16( off + sizeof( int2 ) * 128 );
     int* wsum = reinterpret_cast<int*>( smem + off );
 
     const Real rdx[3] = { (Real)a.mesh.rdx[0], (Real)a.mesh.rdx[1], (Real)a.mesh.rdx[2] };
@@ -901,15 +928,27 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
         const int w = pl.xcd * pl.per_xcd + l;
         return ( l < pl.per_xcd && w < pl.total ) ? w : -1;
     };
+    // This workgroup's part-table entries, 64 at a time, in LDS slots
+    // k & 127 (k = the workgroup's part counter), read by wave 0.
+    auto load_parts = [&]( int k0 ) {
+        if ( threadIdx.x < 64 )
+        {
+            const int k = k0 + (int)threadIdx.x;
+            const int wk = part_at( pl.first + k * pl.stride );
+            ptab[k & 127] = wk >= 0 ? ta.parts[wk] : make_int2( -1, -1 );
+        }
+    };
     int loc = pl.first;
     int w = part_at( loc );
     if ( w < 0 )
         return;
+    int kpart = 0;
 
     // Prologue: the first part's run table and particle loads; zeroed
-    // window and counts.
+    // window and counts; the first 64 part-table entries.
     TileGeom tg;
     part_geometry( ta, a.mesh, w, tg );
+    load_parts( 0 );
     {
         RunLoad rl;
         runs_issue( ta, a.mesh, tg, rl );
@@ -940,7 +979,7 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
         RunLoad rl1 = { 0, 0 };
         if ( w1 >= 0 )
         {
-            part_geometry( ta, a.mesh, w1, tg1 );
+            part_geometry_of( ta, a.mesh, ptab[( kpart + 1 ) & 127], tg1 );
             runs_issue( ta, a.mesh, tg1, rl1 );
         }
         // Base cells of the brick and the node window they reach.
@@ -1021,21 +1060,21 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
         __syncthreads();
         // B: base-cell offsets (cnt[nbt] = sorted particles), then the task
         // table: every base cell's records cut into runs of at most L.
-        const int nsorted = block_exclusive_scan<SBLOCK>( cnt, nbt, wsum );
-        if ( threadIdx.x == 0 )
-            cnt[nbt] = nsorted;
-        __syncthreads();
-        for ( int b = threadIdx.x; b < nbt; b += SBLOCK )
-            sub[b] = ( cnt[b + 1] - cnt[b] + L - 1 ) / L;
-        __syncthreads();
-        const int ntask = block_exclusive_scan<SBLOCK>( sub, nbt, wsum );
+        // One scan of (count | tasks << 16) per base cell gives both the
+        // record offsets (low half) and the task offsets (high half); the
+        // halves cannot carry into each other (a part holds < 2^15 records).
+        const int packed_total = block_exclusive_scan<SBLOCK>(
+            cnt, nbt, wsum, [L]( int c ) { return c | ( ( c + L - 1 ) / L ) << 16; } );
+        const int ntask = packed_total >> 16;
         for ( int b = threadIdx.x; b < nbt; b += SBLOCK )
         {
-            const int c0 = cnt[b], c = cnt[b + 1] - c0;
+            const int e0 = cnt[b];
+            const int e1 = b + 1 < nbt ? cnt[b + 1] : packed_total;
+            const int c0 = e0 & 0xffff, c = ( e1 & 0xffff ) - c0;
             const int lx = b / ( NB[1] * NB[2] );
             const int ly = ( b / NB[2] ) % NB[1];
             const int lz = b % NB[2];
-            int q = sub[b];
+            int q = e0 >> 16;
             for ( int r = 0; r < c; r += L )
                 task[q++] = pack_task( lx, ly, lz, c0 + r, min( L, c - r ) );
         }
@@ -1047,7 +1086,7 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
         {
             if ( key[mi] < 0 )
                 continue;
-            Real* rp = rec + (size_t)RS * ( cnt[key[mi] >> 16] + ( key[mi] & 0xffff ) );
+            Real* rp = rec + (size_t)RS * ( ( cnt[key[mi] >> 16] & 0xffff ) + ( key[mi] & 0xffff ) );
 #pragma unroll
             for ( int c = 0; c < RL::len(); ++c )
                 rp[c] = stash[mi][c];
@@ -1194,6 +1233,8 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
         } );
         for ( int c = threadIdx.x; c <= ta.max_base_cells; c += SBLOCK )
             cnt[c] = 0;
+        if ( ( ( kpart + 2 ) & 63 ) == 0 )
+            load_parts( kpart + 2 );
         // The next part reuses the window, the counts and the tables.
         __syncthreads();
         if ( w1 < 0 )
@@ -1202,6 +1243,7 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
         w = w1;
         tg = tg1;
         buf ^= 1;
+        ++kpart;
     }
 }
 

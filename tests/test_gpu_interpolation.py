@@ -287,6 +287,41 @@ def test_fused_mpm_transfer(ca, oracle, order, dtype, sorted_bins):
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
+def test_fused_mpm_many_parts_per_workgroup(ca, oracle, order, monkeypatch):
+    """Eight workgroups for ~1600 parts (16 particles per cell): every
+    workgroup walks ~200 parts, so the scatter's part-table refills and
+    run-table double buffers cycle many times.  P2G (mass, momentum, stress
+    divergence) against the oracle, FP32-mixed."""
+    monkeypatch.setenv("CABANA_AMD_MAX_TILE_GROUPS", "8")
+    dtype = torch.float32
+    c = Case(ca, oracle, num_cell=(36, 36, 34), n=700000, seed=77 + order)
+    m, q, sig = mpm_fields(c, dtype)
+    gm = c.lg.globalGrid().globalMesh()
+    lo = [gm.lowCorner(d) for d in range(3)]
+    hi = [gm.highCorner(d) for d in range(3)]
+    lcl = ca.createLinkedCellList(c.pts, [gm.cellSize(d) for d in range(3)], lo, hi)
+    perm = lcl.permutes.cpu().numpy()
+    pts = c.pts[lcl.permutes].contiguous()
+    pts_np = c.pts_np[perm]
+    m, q, sig = m[perm], q[perm], sig[perm]
+    tm = torch.tensor(m[:, 0], dtype=dtype, device="cuda")
+    tq = torch.tensor(q, dtype=dtype, device="cuda")
+    ts = torch.tensor(sig.reshape(-1, 3, 3), dtype=dtype, device="cuda")
+    gmass, gmom, gstress = c.array(1, dtype), c.array(3, dtype), c.array(3, dtype)
+    fs = [ca.createScalarValueP2G(tm, 1.0), ca.createVectorValueP2G(tq, 1.0),
+          ca.createTensorDivergenceP2G(ts, -0.25)]
+    ca.p2g(fs, pts, c.n, ca.Spline(order), c.halo(gmass), [gmass, gmom, gstress], bins=lcl)
+    own = c.mesh.owned_slices()
+    for arr, op, vals, mult, nd in ((gmass, 0, m, 1.0, 1), (gmom, 1, q, 1.0, 3),
+                                    (gstress, 4, sig, -0.25, 3)):
+        ref = c.mesh.grid(nd)
+        oracle.p2g(c.mesh, order, op, mult, pts_np, vals, ref)
+        err = rel_l2(arr.view().cpu().numpy()[own], ref[own])
+        assert err <= TOL[dtype], (op, err)
+    assert ca.check_domain() == 0
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
 def test_tiled_scatter_handles_unsorted_and_partial_bins(ca, oracle, order):
     """With a binning that covers only [begin, end) and particles that are
     NOT permuted (so many leave their tile windows), the result still equals
