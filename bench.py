@@ -395,6 +395,26 @@ def cpu_baseline(cells, ppc, order, budget_s):
                       % (n, cells ** 3 * ppc, t)}
 
 
+def copy_bandwidth(torch, mib=2048, reps=5):
+    """Achievable HBM rate on this GPU: y = 2x over `mib` MiB of FP32 (reads
+    and writes `mib` MiB each), timed with HIP events; GB/s."""
+    x = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        torch.mul(x, 2.0, out=y)
+    t1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * (mib << 20) * reps / (t0.elapsed_time(t1) * 1e-3) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -494,6 +514,12 @@ def main():
         out["fp64"] = {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms"]}
         if "fp32" in res:
             out["fp32_over_fp64"] = res["fp32"]["rate"] / res["fp64"]["rate"]
+    copy_gbs = copy_bandwidth(torch)
+    out["hbm_copy"] = {"gbs": copy_gbs,
+                       "probe": "y = 2x, FP32, 2 GiB read + 2 GiB written per launch, "
+                                "torch elementwise kernel, HIP events over 5 launches"}
+    roofline["frac_of_copy"] = roofline["achieved"] / copy_gbs
+    roofline["step_frac_of_copy"] = roofline["step_frac"] * HBM_PEAK_GBS / copy_gbs
     out["sort_ms_per_step"] = head["sort_ms"]
     out["value_with_sort"] = head["n"] / ((head["ms"] + head["sort_ms"]) * 1e-3)
     out["kernels"] = {k: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"]}

@@ -537,6 +537,39 @@ __global__ __launch_bounds__( BLOCK ) void permute_copy_back_kernel(
         *f.ptr<T>( b, c ) = scratch[( i - begin ) * f.ncomp + c];
 }
 
+// Row-contiguous fields (each particle's components in one run of NW 4-byte
+// words, rows row_words apart): one thread per word of the permuted range,
+// so consecutive threads write consecutive scratch words and read
+// consecutive words of each source row.
+template <int NW>
+__global__ __launch_bounds__( BLOCK ) void permute_gather_words_kernel(
+    const uint32_t* src, int64_t row_words, int nw_rt, const int64_t* permutes,
+    int64_t nwords, uint32_t* scratch )
+{
+    const int nw = NW > 0 ? NW : nw_rt;
+    const int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( w >= nwords )
+        return;
+    const int64_t i = w / nw;
+    const int j = (int)( w - i * nw );
+    scratch[w] = src[permutes[i] * row_words + j];
+}
+
+// Copy back into padded rows (row_words > words per particle).
+template <int NW>
+__global__ __launch_bounds__( BLOCK ) void permute_put_words_kernel(
+    uint32_t* dst, int64_t row_words, int nw_rt, int64_t begin, int64_t nwords,
+    const uint32_t* scratch )
+{
+    const int nw = NW > 0 ? NW : nw_rt;
+    const int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( w >= nwords )
+        return;
+    const int64_t i = w / nw;
+    const int j = (int)( w - i * nw );
+    dst[( begin + i ) * row_words + j] = scratch[w];
+}
+
 // Part table of a brick tiling.  One thread per brick counts the brick's
 // particles through the cell offsets of its cell runs and the number of
 // parts of at most t.cap particles (an empty brick has none).
@@ -782,6 +815,50 @@ hipError_t launch_exclusive_scan( Launcher& L, const int32_t* in, int64_t* out,
     L.after( "exclusive_scan", s );
     return hipGetLastError();
 }
+namespace
+{
+// Words per particle of a row-contiguous field, or 0 when the field's
+// components are interleaved with other particles' (vector length > 1).
+int row_field_words( const PFieldDev& f, int elem_size, int64_t* row_words )
+{
+    if ( f.vl != 1 || ( f.ncomp * elem_size ) % 4 != 0 || ( f.stride * elem_size ) % 4 != 0 ||
+         reinterpret_cast<uintptr_t>( f.data ) % 4 != 0 )
+        return 0;
+    *row_words = (int64_t)f.stride * elem_size / 4;
+    return f.ncomp * elem_size / 4;
+}
+
+#define CA_NW_CASES( KER, ... )                                                       \
+    switch ( nw )                                                                     \
+    {                                                                                 \
+    case 1: hipLaunchKernelGGL( KER<1>, grid, dim3( BLOCK ), 0, s, __VA_ARGS__ ); break;   \
+    case 2: hipLaunchKernelGGL( KER<2>, grid, dim3( BLOCK ), 0, s, __VA_ARGS__ ); break;   \
+    case 3: hipLaunchKernelGGL( KER<3>, grid, dim3( BLOCK ), 0, s, __VA_ARGS__ ); break;   \
+    case 4: hipLaunchKernelGGL( KER<4>, grid, dim3( BLOCK ), 0, s, __VA_ARGS__ ); break;   \
+    case 6: hipLaunchKernelGGL( KER<6>, grid, dim3( BLOCK ), 0, s, __VA_ARGS__ ); break;   \
+    case 8: hipLaunchKernelGGL( KER<8>, grid, dim3( BLOCK ), 0, s, __VA_ARGS__ ); break;   \
+    case 9: hipLaunchKernelGGL( KER<9>, grid, dim3( BLOCK ), 0, s, __VA_ARGS__ ); break;   \
+    case 12: hipLaunchKernelGGL( KER<12>, grid, dim3( BLOCK ), 0, s, __VA_ARGS__ ); break; \
+    case 18: hipLaunchKernelGGL( KER<18>, grid, dim3( BLOCK ), 0, s, __VA_ARGS__ ); break; \
+    default: hipLaunchKernelGGL( KER<0>, grid, dim3( BLOCK ), 0, s, __VA_ARGS__ ); break;  \
+    }
+
+void launch_gather_words( int nw, int64_t nwords, hipStream_t s, const uint32_t* src,
+                          int64_t row_words, const int64_t* permutes, uint32_t* scratch )
+{
+    const dim3 grid = blocks_for( nwords );
+    CA_NW_CASES( permute_gather_words_kernel, src, row_words, nw, permutes, nwords, scratch )
+}
+
+void launch_put_words( int nw, int64_t nwords, hipStream_t s, uint32_t* dst, int64_t row_words,
+                       int64_t begin, const uint32_t* scratch )
+{
+    const dim3 grid = blocks_for( nwords );
+    CA_NW_CASES( permute_put_words_kernel, dst, row_words, nw, begin, nwords, scratch )
+}
+#undef CA_NW_CASES
+} // namespace
+
 hipError_t launch_permute_gather( Launcher& L, const PFieldDev& f,
                                   int elem_size, const int64_t* permutes,
                                   int64_t begin, int64_t end, void* scratch,
@@ -790,6 +867,15 @@ hipError_t launch_permute_gather( Launcher& L, const PFieldDev& f,
     if ( end <= begin )
         return hipSuccess;
     L.before( "permute_gather", s );
+    int64_t row_words = 0;
+    if ( const int nw = row_field_words( f, elem_size, &row_words ) )
+    {
+        const int64_t nwords = ( end - begin ) * nw;
+        launch_gather_words( nw, nwords, s, reinterpret_cast<const uint32_t*>( f.data ),
+                             row_words, permutes, reinterpret_cast<uint32_t*>( scratch ) );
+        L.after( "permute_gather", s );
+        return hipGetLastError();
+    }
     if ( elem_size == 8 )
         hipLaunchKernelGGL( permute_gather_kernel<uint64_t>,
                             blocks_for( end - begin ), dim3( BLOCK ), 0, s, f,
@@ -812,6 +898,21 @@ hipError_t launch_permute_copy_back( Launcher& L, const PFieldDev& f,
     if ( end <= begin )
         return hipSuccess;
     L.before( "permute_copy_back", s );
+    int64_t row_words = 0;
+    if ( const int nw = row_field_words( f, elem_size, &row_words ) )
+    {
+        const int64_t nwords = ( end - begin ) * nw;
+        uint32_t* dst = reinterpret_cast<uint32_t*>( f.data );
+        hipError_t e = hipSuccess;
+        if ( row_words == nw )
+            e = hipMemcpyAsync( dst + begin * row_words, scratch, (size_t)nwords * 4,
+                                hipMemcpyDeviceToDevice, s );
+        else
+            launch_put_words( nw, nwords, s, dst, row_words, begin,
+                              reinterpret_cast<const uint32_t*>( scratch ) );
+        L.after( "permute_copy_back", s );
+        return e != hipSuccess ? e : hipGetLastError();
+    }
     if ( elem_size == 8 )
         hipLaunchKernelGGL( permute_copy_back_kernel<uint64_t>,
                             blocks_for( end - begin ), dim3( BLOCK ), 0, s, f,

@@ -87,6 +87,43 @@ def test_linked_cell_sort_shapes(ca, oracle, case):
     assert np.array_equal(lcl.permutes.cpu().numpy(), perm)
 
 
+@pytest.mark.parametrize("begin_end", [(0, None), (777, -1234)])
+def test_permute_row_layouts(ca, oracle, begin_end):
+    """permute of row-contiguous fields (FP64 (n,3), FP32 (n,3,3), int32 (n,))
+    and of a padded-row view ((n,4)[:, :3]) equals indexing by the Serial
+    permutation; entries outside [begin, end) are untouched."""
+    rng = np.random.default_rng(31)
+    n = 50001
+    begin, end = begin_end[0], (n if begin_end[1] is None else n + begin_end[1])
+    pos = rng.uniform(0, 6.0, size=(n, 3))
+    counts, offsets, perm, _ = oracle.lcl_build(pos, [0.5] * 3, [0.0] * 3, [6.0] * 3,
+                                                begin=begin, end=end)
+    lcl = ca.createLinkedCellList(torch.tensor(pos, device="cuda"), [0.5] * 3, [0.0] * 3,
+                                  [6.0] * 3, begin, end)
+    assert np.array_equal(lcl.permutes.cpu().numpy(), perm)
+    a64 = rng.normal(size=(n, 3))
+    a32 = rng.normal(size=(n, 3, 3)).astype(np.float32)
+    ai = rng.integers(-2**31, 2**31 - 1, size=n, dtype=np.int32)
+    wide = rng.normal(size=(n, 4)).astype(np.float32)
+    t64 = torch.tensor(a64, device="cuda")
+    t32 = torch.tensor(a32, device="cuda")
+    ti = torch.tensor(ai, device="cuda")
+    twide = torch.tensor(wide, device="cuda")
+    ca.permute(lcl, t64, t32, ti, twide[:, :3])
+
+    def expect(x):
+        y = x.copy()
+        y[begin:end] = x[perm]
+        return y
+
+    wide_ref = wide.copy()
+    wide_ref[:, :3] = expect(wide[:, :3])
+    assert np.array_equal(t64.cpu().numpy(), expect(a64))
+    assert np.array_equal(t32.cpu().numpy(), expect(a32))
+    assert np.array_equal(ti.cpu().numpy(), expect(ai))
+    assert np.array_equal(twide.cpu().numpy(), wide_ref)
+
+
 def test_linked_cell_aosoa_positions_and_permute(ca, oracle):
     """Positions and a tensor member in one AoSoA (VL = 64): build from the
     slice, permute the slice, compare bitwise with the oracle."""
