@@ -577,30 +577,17 @@ struct RecordLayout
 
 // Lanes per scatter task: the whole stencil (S = 1), one x-slab (S = K) or
 // one (x, y) pencil (S = K * K), the fewest that keep a lane's register
-// accumulator at <= CA_SLAB_ACC values.
-#ifndef CA_SLAB_ACC
-#define CA_SLAB_ACC 64
-#endif
-#ifndef CA_REC_AHEAD
-#define CA_REC_AHEAD 1
-#endif
-#ifndef CA_PART_AHEAD
-#define CA_PART_AHEAD 1
-#endif
-#ifndef CA_SCATTER_BLOCK
-#define CA_SCATTER_BLOCK 256
-#endif
-#ifndef CA_SCATTER_WAVES
-#define CA_SCATTER_WAVES 2
-#endif
+// accumulator at <= SLAB_MAX_ACC values.
+constexpr int SLAB_MAX_ACC = 64;
 // Workgroup size of the scatter and its minimum waves per SIMD (the
 // register budget): two 256-thread workgroups per CU, so one workgroup's
 // memory phases overlap the other's accumulation.
-constexpr int SBLOCK = CA_SCATTER_BLOCK;
+constexpr int SBLOCK = 256;
+constexpr int SCATTER_MIN_WAVES = 2;
 template <int DT>
 struct SlabPlan
 {
-    static constexpr int MAXACC = CA_SLAB_ACC;
+    static constexpr int MAXACC = SLAB_MAX_ACC;
     static constexpr int S =
         ( K * K * K * DT <= MAXACC ) ? 1 : ( ( K * K * DT <= MAXACC ) ? K : K * K );
     static constexpr int NI = ( S == 1 ) ? K : 1;
@@ -780,11 +767,14 @@ __device__ __forceinline__ unsigned pack_task( int lx, int ly, int lz, int start
 }
 
 // The run table of a brick in two halves, so its global reads can be issued
-// early and committed later: lane r < nruns (<= 64) of wave 0 holds run r.
+// early and committed later: lane r < nruns (<= 64) of wave 0 holds run r's
+// raw reads (nothing is computed from them until the commit, so issuing
+// them does not wait for them).
 struct RunLoad
 {
     int64_t start;
-    int64_t count;
+    int64_t last_offset;
+    int32_t last_count;
 };
 
 __device__ __forceinline__ void runs_issue( const TileArgs& ta, const MeshDev& m,
@@ -794,14 +784,16 @@ __device__ __forceinline__ void runs_issue( const TileArgs& ta, const MeshDev& m
     const int nruns = ( g.c1[0] - g.c0[0] ) * nj;
     const int r = threadIdx.x;
     rl.start = 0;
-    rl.count = 0;
+    rl.last_offset = 0;
+    rl.last_count = 0;
     if ( r < nruns )
     {
         const int ci = g.c0[0] + r / nj, cj = g.c0[1] + r % nj;
         const int64_t cfirst = ( (int64_t)ci * m.num_cell[1] + cj ) * m.num_cell[2] + g.c0[2];
         const int64_t clast = cfirst + ( g.c1[2] - g.c0[2] ) - 1;
         rl.start = ta.cell_offsets[cfirst];
-        rl.count = ta.cell_offsets[clast] + ta.cell_counts[clast] - rl.start;
+        rl.last_offset = ta.cell_offsets[clast];
+        rl.last_count = ta.cell_counts[clast];
     }
 }
 
@@ -814,7 +806,7 @@ __device__ __forceinline__ void runs_commit( const RunLoad& rl, const TileGeom& 
         return;
     const int nruns = ( g.c1[0] - g.c0[0] ) * ( g.c1[1] - g.c0[1] );
     const int lane = threadIdx.x;
-    int v = (int)rl.count;
+    int v = (int)( rl.last_offset + rl.last_count - rl.start );
 #pragma unroll
     for ( int o = 1; o < 64; o <<= 1 )
     {
@@ -881,7 +873,7 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
 }
 
 template <class Real, bool CONTIG, bool SHARED, int... FORMS>
-__global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kernel(
+__global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_kernel(
     ScatterArgs a, TileArgs ta )
 {
     using FL = FormList<FORMS...>;
@@ -961,7 +953,6 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
     __syncthreads();
     int buf = 0;
     PartPrefetch<Real, MAXM, NPT> pf;
-    if ( CA_PART_AHEAD )
     {
         const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
         const int total = run_pre[0][nruns];
@@ -976,12 +967,9 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
         const int loc1 = loc + pl.stride;
         const int w1 = part_at( loc1 );
         TileGeom tg1 = tg;
-        RunLoad rl1 = { 0, 0 };
+        RunLoad rl1 = { 0, 0, 0 };
         if ( w1 >= 0 )
-        {
             part_geometry_of( ta, a.mesh, ptab[( kpart + 1 ) & 127], tg1 );
-            runs_issue( ta, a.mesh, tg1, rl1 );
-        }
         // Base cells of the brick and the node window they reach.
         int blo[3], NB[3];
 #pragma unroll
@@ -991,15 +979,6 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
             NB[d] = ( tg.c1[d] - tg.c0[d] ) + BASE_SPREAD;
         }
         const int nbt = NB[0] * NB[1] * NB[2];
-        if ( !CA_PART_AHEAD )
-        {
-            const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
-            const int total = run_pre[buf][nruns];
-            const int pbeg = tg.part * ta.cap;
-            prefetch_part<Real, CONTIG, MAXM, SBLOCK, FORMS...>( a, ta, run_start[buf], run_pre[buf],
-                                                         nruns, pbeg, min( ta.cap, total - pbeg ),
-                                                         pf );
-        }
 
         // A: stencil base cell of every particle of the part (from the
         // prefetched registers), counted per base cell (the slot is the
@@ -1057,6 +1036,9 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
                 global_scatter_particle<Real, SHARED, FORMS...>( a, p, sp );
             }
         }
+        // The next part's run table reads, committed after B.
+        if ( w1 >= 0 )
+            runs_issue( ta, a.mesh, tg1, rl1 );
         __syncthreads();
         // B: base-cell offsets (cnt[nbt] = sorted particles), then the task
         // table: every base cell's records cut into runs of at most L.
@@ -1094,7 +1076,7 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
         __syncthreads();
         // The next part's particle reads are issued here and complete while
         // this part accumulates.
-        if ( CA_PART_AHEAD && w1 >= 0 )
+        if ( w1 >= 0 )
         {
             const int nruns1 = ( tg1.c1[0] - tg1.c0[0] ) * ( tg1.c1[1] - tg1.c0[1] );
             const int total1 = run_pre[buf ^ 1][nruns1];
@@ -1148,7 +1130,6 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
 #pragma unroll
                         for ( int d = 0; d < DG; ++d )
                             acc[n][d] = Real( 0 );
-#if CA_REC_AHEAD
                     Real cur[RLEN];
 #pragma unroll
                     for ( int c = 0; c < RLEN; ++c )
@@ -1165,16 +1146,6 @@ __global__ __launch_bounds__( SBLOCK, CA_SCATTER_WAVES ) void scatter_cells_kern
                         for ( int c = 0; c < RLEN; ++c )
                             cur[c] = nxt[c];
                     }
-#else
-                    for ( int r = r0; r < r1; ++r )
-                    {
-                        Real cur[RLEN];
-#pragma unroll
-                        for ( int c = 0; c < RLEN; ++c )
-                            cur[c] = rec[(size_t)RS * r + c];
-                        scatter_record<Real, SHARED, grp, S, FORMS...>( cur, rdx, sx, sy, acc );
-                    }
-#endif
                     double* wb = win + ( ( lx + sx ) * W[1] + ly + sy ) * W[2] * DT + lz * DT;
 #pragma unroll
                     for ( int i = 0; i < NI; ++i )
