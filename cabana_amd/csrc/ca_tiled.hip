@@ -673,7 +673,6 @@ __device__ __forceinline__ void scatter_record(
     constexpr int DG = TG::dofs( G );
     constexpr int NI = ( S == 1 ) ? K : 1;
     constexpr int NJ = ( S == K * K ) ? 1 : K;
-    constexpr bool GRAD = TG::any_grad( G );
     Spline1D<ORDER, Real> sp[3];
 #pragma unroll
     for ( int d = 0; d < 3; ++d )
@@ -734,27 +733,34 @@ __device__ __forceinline__ void scatter_record(
                 slab_coeffs<FL::v[t]>( rp + RL::attr( t ), wx[i], gx[i], u + O, v + O, w + O );
             }
         } );
-        // (j, k) products, formed where they are used: A always, B and C
-        // for gradient-type forms.
+        // Node (j, k) of the slab gets wy wz u + gy wz v + wy gz w
+        // = wz (wy u + gy v) + gz (wy w): per row j, P = wy u + gy v and
+        // Q = wy w (value-type dofs: P = wy u, Q = 0), then two FMAs per
+        // node and dof.
 #pragma unroll
         for ( int j = 0; j < NJ; ++j )
+        {
+            Real P[DG], Q[DG];
+            static_for<DG>( [&]( auto D ) {
+                constexpr int d = D;
+                if constexpr ( TG::grad_dof( G, d ) )
+                {
+                    P[d] = fma_r( gy[j], v[d], wy[j] * u[d] );
+                    Q[d] = wy[j] * w[d];
+                }
+                else
+                    P[d] = wy[j] * u[d];
+            } );
 #pragma unroll
             for ( int k = 0; k < K; ++k )
-            {
-                const Real A = wy[j] * sp[2].w[k];
-                const Real B = GRAD ? gy[j] * sp[2].w[k] : Real( 0 );
-                const Real C = GRAD ? wy[j] * sp[2].g[k] : Real( 0 );
                 static_for<DG>( [&]( auto D ) {
                     constexpr int d = D;
                     Real& c = acc[( i * NJ + j ) * K + k][d];
-                    c = fma_r( A, u[d], c );
+                    c = fma_r( sp[2].w[k], P[d], c );
                     if constexpr ( TG::grad_dof( G, d ) )
-                    {
-                        c = fma_r( B, v[d], c );
-                        c = fma_r( C, w[d], c );
-                    }
+                        c = fma_r( sp[2].g[k], Q[d], c );
                 } );
-            }
+        }
     }
 }
 
