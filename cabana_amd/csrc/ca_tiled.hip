@@ -853,6 +853,7 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
 #pragma unroll
     for ( int mi = 0; mi < MAXM; ++mi )
         run[mi] = 0;
+#pragma unroll 8
     for ( int q = 1; q < nruns; ++q )
     {
         const int rp = run_pre[q];
@@ -1181,33 +1182,41 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
         }
         __syncthreads();
 
-        // E: flush the window, one global atomic per non-zero node value,
-        // walking the window with the grid's unit-stride dimension fastest;
-        // every entry is zeroed by the thread that flushed it, and the
-        // counts are cleared, for the next part.
-        static_for<( SHARED ? 1 : NT )>( [&]( auto I ) {
-            constexpr int t = I;
-            constexpr int NG = SHARED ? 3 : ng_of( FL::v[t] );
-            constexpr int O = SD::offset( t, SHARED );
-            const GFieldDev& g = a.t[t].grid;
-            for ( int f = threadIdx.x; f < wnodes * NG; f += SBLOCK )
+        // E: flush the window, one thread per node (the grid's unit-stride
+        // dimension fastest across threads): the node's dofs are read and
+        // zeroed for the next part, and every non-zero value is added to its
+        // grid with one global atomic.
+        for ( int f = threadIdx.x; f < wnodes; f += SBLOCK )
+        {
+            int q[3], d0;
+            window_coords<1>( ta, f, W, rW, q, d0 );
+            double* src = win + ( ( q[0] * W[1] + q[1] ) * W[2] + q[2] ) * DT;
+            double v[DT];
+#pragma unroll
+            for ( int d = 0; d < DT; ++d )
             {
-                int q[3], d;
-                window_coords<NG>( ta, f, W, rW, q, d );
-                double* src = win + ( ( q[0] * W[1] + q[1] ) * W[2] + q[2] ) * DT + O + d;
-                const Real v = (Real)*src;
-                *src = 0.0;
-                if ( v == Real( 0 ) )
-                    continue;
-                const int gi = blo[0] + q[0], gj = blo[1] + q[1], gk = blo[2] + q[2];
-                if ( gi < 0 || gj < 0 || gk < 0 || gi >= a.mesh.num_node[0] ||
-                     gj >= a.mesh.num_node[1] || gk >= a.mesh.num_node[2] )
-                    continue;
-                atomic_add( gptr<Real>( g, gi * g.s[0] + gj * g.s[1] + gk * g.s[2] +
-                                               d * g.s[3] ),
-                            v );
+                v[d] = src[d];
+                src[d] = 0.0;
             }
-        } );
+            const int gi = blo[0] + q[0], gj = blo[1] + q[1], gk = blo[2] + q[2];
+            if ( gi < 0 || gj < 0 || gk < 0 || gi >= a.mesh.num_node[0] ||
+                 gj >= a.mesh.num_node[1] || gk >= a.mesh.num_node[2] )
+                continue;
+            static_for<( SHARED ? 1 : NT )>( [&]( auto I ) {
+                constexpr int t = I;
+                constexpr int NG = SHARED ? 3 : ng_of( FL::v[t] );
+                constexpr int O = SD::offset( t, SHARED );
+                const GFieldDev& g = a.t[t].grid;
+                Real* dst = gptr<Real>( g, gi * g.s[0] + gj * g.s[1] + gk * g.s[2] );
+#pragma unroll
+                for ( int d = 0; d < NG; ++d )
+                {
+                    const Real x = (Real)v[O + d];
+                    if ( x != Real( 0 ) )
+                        atomic_add( dst + d * g.s[3], x );
+                }
+            } );
+        }
         for ( int c = threadIdx.x; c <= ta.max_base_cells; c += SBLOCK )
             cnt[c] = 0;
         if ( ( ( kpart + 2 ) & 63 ) == 0 )
