@@ -321,6 +321,60 @@ def test_fused_mpm_many_parts_per_workgroup(ca, oracle, order, monkeypatch):
     assert ca.check_domain() == 0
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_tiled_layout_left_grids_and_aosoa_attributes(ca, oracle, order, dtype):
+    """Cell-sorted (tiled) MPM P2G and G2P with LayoutLeft grids and the
+    particle attributes in one AoSoA (VL = 16: the interleaved path)."""
+    c = Case(ca, oracle, num_cell=(12, 10, 9), n=20000, seed=41 + order)
+    m, q, sig = mpm_fields(c, dtype)
+    gm = c.lg.globalGrid().globalMesh()
+    lo = [gm.lowCorner(d) for d in range(3)]
+    hi = [gm.highCorner(d) for d in range(3)]
+    lcl = ca.createLinkedCellList(c.pts, [gm.cellSize(d) for d in range(3)], lo, hi)
+    perm = lcl.permutes.cpu().numpy()
+    pts = c.pts[lcl.permutes].contiguous()
+    pts_np = c.pts_np[perm]
+    m, q, sig = m[perm], q[perm], sig[perm]
+    vl = 16
+    nsoa = (c.n + vl - 1) // vl
+    stride = vl * (1 + 3 + 9)
+    raw = torch.zeros(nsoa * stride, dtype=dtype, device="cuda")
+    sm = ca.AoSoASlice(raw[0:], c.n, (), vl, stride)
+    sq = ca.AoSoASlice(raw[vl:], c.n, (3,), vl, stride)
+    ss = ca.AoSoASlice(raw[4 * vl:], c.n, (3, 3), vl, stride)
+    sm.from_tensor(torch.tensor(m[:, 0], device="cuda"))
+    sq.from_tensor(torch.tensor(q, device="cuda"))
+    ss.from_tensor(torch.tensor(sig.reshape(-1, 3, 3), device="cuda"))
+    gmass = c.array(1, dtype, memory_layout="left")
+    gmom = c.array(3, dtype, memory_layout="left")
+    gstress = c.array(3, dtype, memory_layout="left")
+    fs = [ca.createScalarValueP2G(sm, 1.0), ca.createVectorValueP2G(sq, 1.0),
+          ca.createTensorDivergenceP2G(ss, -0.25)]
+    ca.p2g(fs, pts, c.n, ca.Spline(order), c.halo(gmass), [gmass, gmom, gstress], bins=lcl)
+    own = c.mesh.owned_slices()
+    for arr, op, vals, mult, nd in ((gmass, 0, m, 1.0, 1), (gmom, 1, q, 1.0, 3),
+                                    (gstress, 4, sig, -0.25, 3)):
+        ref = c.mesh.grid(nd)
+        oracle.p2g(c.mesh, order, op, mult, pts_np, vals, ref)
+        assert rel_l2(arr.view().cpu().numpy()[own], ref[own]) <= TOL[dtype], op
+    u = c.array(3, dtype, memory_layout="left")
+    un = c.rng.normal(size=u.view().shape)
+    u.view().copy_(torch.tensor(un, dtype=dtype))
+    vel = torch.zeros(c.n, 3, dtype=dtype, device="cuda")
+    grad = torch.zeros(c.n, 3, 3, dtype=dtype, device="cuda")
+    ca.g2p([u, u], c.halo(u), pts, c.n, ca.Spline(order),
+           [ca.createVectorValueG2P(vel, 1.0), ca.createVectorGradientG2P(grad, 1.0)],
+           bins=lcl)
+    vref = np.zeros((c.n, 3))
+    gref = np.zeros((c.n, 9))
+    oracle.g2p(c.mesh, order, 1, 1.0, un.copy(), pts_np, vref)
+    oracle.g2p(c.mesh, order, 3, 1.0, un.copy(), pts_np, gref)
+    assert rel_l2(vel.cpu().numpy(), vref) <= TOL[dtype]
+    assert rel_l2(grad.cpu().numpy().reshape(-1, 9), gref) <= TOL[dtype]
+    assert ca.check_domain() == 0
+
+
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_tiled_scatter_handles_unsorted_and_partial_bins(ca, oracle, order):
     """With a binning that covers only [begin, end) and particles that are
