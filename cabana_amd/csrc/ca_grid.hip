@@ -541,18 +541,37 @@ __global__ __launch_bounds__( BLOCK ) void permute_copy_back_kernel(
 // words, rows row_words apart): one thread per word of the permuted range,
 // so consecutive threads write consecutive scratch words and read
 // consecutive words of each source row.
+// Words per thread of the word-parallel permute kernels: a workgroup moves
+// PERMUTE_WPT * BLOCK consecutive words, thread t the words t + k * BLOCK,
+// so every access of the wave stays coalesced while each thread keeps
+// 2 * PERMUTE_WPT independent loads in flight.
+constexpr int PERMUTE_WPT = 8;
+
 template <int NW>
 __global__ __launch_bounds__( BLOCK ) void permute_gather_words_kernel(
     const uint32_t* src, int64_t row_words, int nw_rt, const int64_t* permutes,
     int64_t nwords, uint32_t* scratch )
 {
     const int nw = NW > 0 ? NW : nw_rt;
-    const int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if ( w >= nwords )
-        return;
-    const int64_t i = w / nw;
-    const int j = (int)( w - i * nw );
-    scratch[w] = src[permutes[i] * row_words + j];
+    // 32-bit index math (the host keeps nwords below 2^31).
+    const uint32_t w0 = (uint32_t)blockIdx.x * ( PERMUTE_WPT * BLOCK ) + threadIdx.x;
+    int64_t srcw[PERMUTE_WPT];
+#pragma unroll
+    for ( int k = 0; k < PERMUTE_WPT; ++k )
+    {
+        const uint32_t w = w0 + k * BLOCK;
+        const uint32_t i = w / (uint32_t)nw;
+        const int j = (int)( w - i * (uint32_t)nw );
+        srcw[k] = w < nwords ? permutes[i] * row_words + j : -1;
+    }
+    uint32_t v[PERMUTE_WPT];
+#pragma unroll
+    for ( int k = 0; k < PERMUTE_WPT; ++k )
+        v[k] = srcw[k] >= 0 ? src[srcw[k]] : 0u;
+#pragma unroll
+    for ( int k = 0; k < PERMUTE_WPT; ++k )
+        if ( srcw[k] >= 0 )
+            scratch[w0 + k * BLOCK] = v[k];
 }
 
 // Copy back into padded rows (row_words > words per particle).
@@ -565,9 +584,10 @@ __global__ __launch_bounds__( BLOCK ) void permute_put_words_kernel(
     const int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if ( w >= nwords )
         return;
-    const int64_t i = w / nw;
-    const int j = (int)( w - i * nw );
-    dst[( begin + i ) * row_words + j] = scratch[w];
+    const uint32_t w32 = (uint32_t)w;
+    const uint32_t i = w32 / (uint32_t)nw;
+    const int j = (int)( w32 - i * (uint32_t)nw );
+    dst[( begin + (int64_t)i ) * row_words + j] = scratch[w];
 }
 
 // Part table of a brick tiling.  One thread per brick counts the brick's
@@ -846,7 +866,7 @@ int row_field_words( const PFieldDev& f, int elem_size, int64_t* row_words )
 void launch_gather_words( int nw, int64_t nwords, hipStream_t s, const uint32_t* src,
                           int64_t row_words, const int64_t* permutes, uint32_t* scratch )
 {
-    const dim3 grid = blocks_for( nwords );
+    const dim3 grid( (unsigned)( ( nwords + PERMUTE_WPT * BLOCK - 1 ) / ( PERMUTE_WPT * BLOCK ) ) );
     CA_NW_CASES( permute_gather_words_kernel, src, row_words, nw, permutes, nwords, scratch )
 }
 
@@ -868,8 +888,10 @@ hipError_t launch_permute_gather( Launcher& L, const PFieldDev& f,
         return hipSuccess;
     L.before( "permute_gather", s );
     int64_t row_words = 0;
-    if ( const int nw = row_field_words( f, elem_size, &row_words ) )
+    const int nwg = row_field_words( f, elem_size, &row_words );
+    if ( nwg && ( end - begin ) * nwg < ( int64_t( 1 ) << 31 ) )
     {
+        const int nw = nwg;
         const int64_t nwords = ( end - begin ) * nw;
         launch_gather_words( nw, nwords, s, reinterpret_cast<const uint32_t*>( f.data ),
                              row_words, permutes, reinterpret_cast<uint32_t*>( scratch ) );
@@ -899,8 +921,10 @@ hipError_t launch_permute_copy_back( Launcher& L, const PFieldDev& f,
         return hipSuccess;
     L.before( "permute_copy_back", s );
     int64_t row_words = 0;
-    if ( const int nw = row_field_words( f, elem_size, &row_words ) )
+    const int nwp = row_field_words( f, elem_size, &row_words );
+    if ( nwp && ( end - begin ) * nwp < ( int64_t( 1 ) << 31 ) )
     {
+        const int nw = nwp;
         const int64_t nwords = ( end - begin ) * nw;
         uint32_t* dst = reinterpret_cast<uint32_t*>( f.data );
         hipError_t e = hipSuccess;
