@@ -87,6 +87,30 @@ def test_linked_cell_sort_shapes(ca, oracle, case):
     assert np.array_equal(lcl.permutes.cpu().numpy(), perm)
 
 
+@pytest.mark.parametrize("case", ["empty_range", "single_bin", "four_passes"])
+def test_linked_cell_sort_edges(ca, oracle, case):
+    """An empty [begin, end), a grid of one bin (one-bit keys) and a grid of
+    more than 2^24 bins (four radix passes): counts, offsets and permutes
+    equal the Serial build."""
+    rng = np.random.default_rng({"empty_range": 4, "single_bin": 5, "four_passes": 6}[case])
+    if case == "four_passes":
+        nx = (256, 256, 257)
+        n = 100000
+    else:
+        nx = (1, 1, 1) if case == "single_bin" else (4, 4, 4)
+        n = 5000
+    lo = np.zeros(3)
+    hi = np.array(nx, dtype=np.float64)
+    pos = rng.uniform(0, 1, size=(n, 3)) * hi
+    begin, end = (2500, 2500) if case == "empty_range" else (0, n)
+    counts, offsets, perm, _ = oracle.lcl_build(pos, [1.0] * 3, lo, hi, begin=begin, end=end)
+    lcl = ca.createLinkedCellList(torch.tensor(pos, device="cuda"), [1.0] * 3, lo, hi,
+                                  begin, end)
+    assert np.array_equal(lcl.counts.cpu().numpy(), counts)
+    assert np.array_equal(lcl.offsets.cpu().numpy(), offsets)
+    assert np.array_equal(lcl.permutes.cpu().numpy()[:end - begin], perm[:end - begin])
+
+
 @pytest.mark.parametrize("begin_end", [(0, None), (777, -1234)])
 def test_permute_row_layouts(ca, oracle, begin_end):
     """permute of row-contiguous fields (FP64 (n,3), FP32 (n,3,3), int32 (n,))
