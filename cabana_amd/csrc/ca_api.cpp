@@ -483,7 +483,8 @@ int64_t max_tile_groups()
         x = 4096;
     return (int64_t)( x / 8 * 8 );
 }
-constexpr int SCATTER_SUB_LEN = 12;
+// Longest run of one base cell that one scatter lane accumulates.
+constexpr int SCATTER_SUB_LEN = 9;
 
 struct TileShape
 {

@@ -1048,7 +1048,8 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
             runs_issue( ta, a.mesh, tg1, rl1 );
         __syncthreads();
         // B: base-cell offsets (cnt[nbt] = sorted particles), then the task
-        // table: every base cell's records cut into runs of at most L.
+        // table: every base cell's c records cut into ceil(c / L) runs of
+        // near-equal length (at most L; a wave waits for its longest run).
         // One scan of (count | tasks << 16) per base cell gives both the
         // record offsets (low half) and the task offsets (high half); the
         // halves cannot carry into each other (a part holds < 2^15 records).
@@ -1064,8 +1065,14 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
             const int ly = ( b / NB[2] ) % NB[1];
             const int lz = b % NB[2];
             int q = e0 >> 16;
-            for ( int r = 0; r < c; r += L )
-                task[q++] = pack_task( lx, ly, lz, c0 + r, min( L, c - r ) );
+            const int nt = ( c + L - 1 ) / L;
+            const int len = nt > 0 ? c / nt : 0, longer = nt > 0 ? c - len * nt : 0;
+            for ( int i = 0, r = 0; i < nt; ++i )
+            {
+                const int n = len + ( i < longer ? 1 : 0 );
+                task[q++] = pack_task( lx, ly, lz, c0 + r, n );
+                r += n;
+            }
         }
         if ( w1 >= 0 )
             runs_commit( rl1, tg1, run_start[buf ^ 1], run_pre[buf ^ 1] );
