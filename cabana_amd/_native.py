@@ -145,6 +145,8 @@ SIGNATURES = {
                                                c_void_p, c_void_p, c_void_p, c_void_p]),
     "cabana_amd_permute": (c_int32, [c_void_p, c_void_p, c_int64, c_int64,
                                      P(ParticleField), c_void_p]),
+    "cabana_amd_permute_members": (c_int32, [c_void_p, c_void_p, c_int64, c_int64,
+                                             P(ParticleField), c_int32, c_void_p]),
     "cabana_amd_profile_enable": (c_int32, [c_void_p, c_int32]),
     "cabana_amd_profile_reset": (c_int32, [c_void_p]),
     "cabana_amd_profile_count": (c_int32, [c_void_p, P(c_int32)]),

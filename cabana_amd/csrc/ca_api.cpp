@@ -1587,6 +1587,41 @@ extern "C" cabana_amd_status cabana_amd_permute(
     return CABANA_AMD_OK;
 }
 
+extern "C" cabana_amd_status cabana_amd_permute_members(
+    cabana_amd_context* ctx, const int64_t* permutes, int64_t begin, int64_t end,
+    const cabana_amd_particle_field* fields, int32_t num_fields,
+    cabana_amd_stream stream )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    if ( num_fields < 0 || ( num_fields > 0 && !fields ) || ( !permutes && end > begin ) )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "NULL argument" );
+    std::vector<PFieldDev> f( (size_t)num_fields );
+    std::vector<int> es( (size_t)num_fields );
+    size_t row_bytes = 0;
+    for ( int k = 0; k < num_fields; ++k )
+    {
+        if ( begin < 0 || end < begin || end > fields[k].num_particle )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "begin/end outside the particle range" );
+        es[k] = dtype_size( fields[k].dtype );
+        if ( !es[k] )
+            return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED, "unknown dtype" );
+        CA_CHECK( make_pfield( ctx, fields[k], -1, -1, end, false, "field", f[k] ) );
+        row_bytes += (size_t)es[k] * f[k].ncomp;
+    }
+    if ( end <= begin || num_fields == 0 )
+        return CABANA_AMD_OK;
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    CA_HIP( ctx, ctx->ensure( SLOT_PERMUTE, row_bytes * (size_t)( end - begin + 1 ) ),
+            "scratch" );
+    CA_HIP( ctx,
+            launch_permute_members( *ctx, f.data(), es.data(), num_fields, permutes, begin,
+                                    end, ctx->scratch[SLOT_PERMUTE], s ),
+            "permute members" );
+    return CABANA_AMD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Profiling
 // ---------------------------------------------------------------------------

@@ -574,6 +574,48 @@ __global__ __launch_bounds__( BLOCK ) void permute_gather_words_kernel(
             scratch[w0 + k * BLOCK] = v[k];
 }
 
+// All row-contiguous members of a particle set in one pass: a workgroup
+// takes PERMUTE_CHUNK consecutive destination particles, reads their
+// permutation entries once into LDS, then copies each member's words of the
+// chunk into that member's block of the scratch (member-major: member f at
+// scratch + n * word_offset[f]), word-parallel and coalesced as in
+// permute_gather_words_kernel.
+constexpr int PERMUTE_CHUNK = 512;
+
+__global__ __launch_bounds__( BLOCK ) void permute_gather_members_kernel(
+    PermuteMembers pm, const int64_t* permutes, int64_t n, uint32_t* scratch )
+{
+    __shared__ int64_t perm[PERMUTE_CHUNK];
+    const int64_t i0 = (int64_t)blockIdx.x * PERMUTE_CHUNK;
+    const int np = (int)min( (int64_t)PERMUTE_CHUNK, n - i0 );
+    for ( int i = threadIdx.x; i < np; i += BLOCK )
+        perm[i] = permutes[i0 + i];
+    __syncthreads();
+    for ( int f = 0; f < pm.count; ++f )
+    {
+        const uint32_t* src = pm.src[f];
+        const int64_t rw = pm.row_words[f];
+        const uint32_t nw = (uint32_t)pm.words[f];
+        uint32_t* dst = scratch + n * pm.word_offset[f] + i0 * nw;
+        const uint32_t nwords = (uint32_t)np * nw;
+        for ( uint32_t w0 = threadIdx.x; w0 < nwords; w0 += PERMUTE_WPT * BLOCK )
+        {
+            uint32_t v[PERMUTE_WPT];
+#pragma unroll
+            for ( int k = 0; k < PERMUTE_WPT; ++k )
+            {
+                const uint32_t w = w0 + k * BLOCK;
+                const uint32_t i = w / nw;
+                v[k] = w < nwords ? src[perm[i] * rw + ( w - i * nw )] : 0u;
+            }
+#pragma unroll
+            for ( int k = 0; k < PERMUTE_WPT; ++k )
+                if ( w0 + k * BLOCK < nwords )
+                    dst[w0 + k * BLOCK] = v[k];
+        }
+    }
+}
+
 // Copy back into padded rows (row_words > words per particle).
 template <int NW>
 __global__ __launch_bounds__( BLOCK ) void permute_put_words_kernel(
@@ -949,6 +991,81 @@ hipError_t launch_permute_copy_back( Launcher& L, const PFieldDev& f,
                             reinterpret_cast<const uint32_t*>( scratch ) );
     L.after( "permute_copy_back", s );
     return hipGetLastError();
+}
+
+hipError_t launch_permute_members( Launcher& L, const PFieldDev* f,
+                                   const int* elem_size, int count,
+                                   const int64_t* permutes, int64_t begin,
+                                   int64_t end, void* scratch, hipStream_t s )
+{
+    const int64_t n = end - begin;
+    if ( n <= 0 || count <= 0 )
+        return hipSuccess;
+    // Members that fit the fused pass: row-contiguous, at most
+    // PERMUTE_MAX_MEMBERS of them, each with n * words below 2^31.
+    PermuteMembers pm{};
+    int index[PERMUTE_MAX_MEMBERS];
+    int64_t words = 0;
+    for ( int k = 0; k < count && pm.count < PERMUTE_MAX_MEMBERS; ++k )
+    {
+        int64_t rw = 0;
+        const int nw = row_field_words( f[k], elem_size[k], &rw );
+        if ( !nw || n * nw >= ( int64_t( 1 ) << 31 ) )
+            continue;
+        const int m = pm.count++;
+        index[m] = k;
+        pm.src[m] = reinterpret_cast<const uint32_t*>( f[k].data );
+        pm.row_words[m] = rw;
+        pm.words[m] = nw;
+        pm.word_offset[m] = words;
+        words += nw;
+    }
+    uint32_t* scr = reinterpret_cast<uint32_t*>( scratch );
+    if ( pm.count >= 2 )
+    {
+        L.before( "permute_gather", s );
+        const dim3 grid( (unsigned)( ( n + PERMUTE_CHUNK - 1 ) / PERMUTE_CHUNK ) );
+        hipLaunchKernelGGL( permute_gather_members_kernel, grid, dim3( BLOCK ), 0, s, pm,
+                            permutes, n, scr );
+        L.after( "permute_gather", s );
+        hipError_t e = hipGetLastError();
+        if ( e != hipSuccess )
+            return e;
+        for ( int m = 0; m < pm.count; ++m )
+        {
+            L.before( "permute_copy_back", s );
+            const int nw = pm.words[m];
+            const int64_t rw = pm.row_words[m];
+            uint32_t* dst = reinterpret_cast<uint32_t*>( f[index[m]].data );
+            const uint32_t* blk = scr + n * pm.word_offset[m];
+            if ( rw == nw )
+                e = hipMemcpyAsync( dst + begin * rw, blk, (size_t)( n * nw ) * 4,
+                                    hipMemcpyDeviceToDevice, s );
+            else
+                launch_put_words( nw, n * nw, s, dst, rw, begin, blk );
+            L.after( "permute_copy_back", s );
+            if ( e == hipSuccess )
+                e = hipGetLastError();
+            if ( e != hipSuccess )
+                return e;
+        }
+    }
+    // The other members one at a time (the scratch is reused in stream order).
+    for ( int k = 0; k < count; ++k )
+    {
+        bool done = false;
+        for ( int m = 0; m < pm.count && pm.count >= 2; ++m )
+            done = done || index[m] == k;
+        if ( done )
+            continue;
+        hipError_t e = launch_permute_gather( L, f[k], elem_size[k], permutes, begin, end,
+                                              scratch, s );
+        if ( e == hipSuccess )
+            e = launch_permute_copy_back( L, f[k], elem_size[k], begin, end, scratch, s );
+        if ( e != hipSuccess )
+            return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_fill_i32( Launcher& L, int32_t* dst, int64_t n,

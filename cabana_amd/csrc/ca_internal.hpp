@@ -409,6 +409,25 @@ hipError_t launch_lcl_sort( Launcher& L, int64_t n, int key_bits,
 hipError_t launch_lcl_bounds( Launcher& L, const int32_t* sorted_keys,
                               int64_t n, int64_t ncell, int32_t* counts,
                               int64_t* offsets, hipStream_t s );
+// Row-contiguous members of one particle set, permuted in one fused pass:
+// member f's rows are row_words[f] 4-byte words apart, words[f] of them per
+// particle, and its block of the scratch starts at n * word_offset[f] words.
+constexpr int PERMUTE_MAX_MEMBERS = 16;
+struct PermuteMembers
+{
+    int32_t count;
+    int32_t words[PERMUTE_MAX_MEMBERS];
+    const uint32_t* src[PERMUTE_MAX_MEMBERS];
+    int64_t row_words[PERMUTE_MAX_MEMBERS];
+    int64_t word_offset[PERMUTE_MAX_MEMBERS];
+};
+// Permutes count members in place (field[i] = old field[permutes[i - begin]]
+// for i in [begin, end)); scratch holds at least
+// sum over members of (elem_size * ncomp) * (end - begin + 1) bytes.
+hipError_t launch_permute_members( Launcher& L, const PFieldDev* f,
+                                   const int* elem_size, int count,
+                                   const int64_t* permutes, int64_t begin,
+                                   int64_t end, void* scratch, hipStream_t s );
 hipError_t launch_permute_gather( Launcher& L, const PFieldDev& f,
                                   int elem_size, const int64_t* permutes,
                                   int64_t begin, int64_t end, void* scratch,

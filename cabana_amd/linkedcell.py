@@ -89,11 +89,22 @@ def createLinkedCellList(positions, grid_delta, grid_min, grid_max, begin=0, end
 
 
 def permute(linked_cell_list, *fields):
-    """permute(linked_cell_list, slice) for one or more particle fields."""
+    """permute(linked_cell_list, slice) for one field; for several,
+    permute(linked_cell_list, aosoa) over those members: each is permuted
+    once, in one fused pass where their layout allows.  The members must not
+    share memory (passing one field twice is rejected)."""
     ctx = N.context()
-    for f in fields:
-        d = particle_descriptor(f)
+    perm = (linked_cell_list.permutes.data_ptr() if linked_cell_list.permutes.numel()
+            else None)
+    descs = [particle_descriptor(f) for f in fields]
+    if len(descs) == 1:
         N.check(N.load().cabana_amd_permute(
-            ctx, linked_cell_list.permutes.data_ptr() if linked_cell_list.permutes.numel()
-            else None, linked_cell_list.rangeBegin(), linked_cell_list.rangeEnd(),
-            ctypes.byref(d), N.current_stream()), ctx)
+            ctx, perm, linked_cell_list.rangeBegin(), linked_cell_list.rangeEnd(),
+            ctypes.byref(descs[0]), N.current_stream()), ctx)
+        return
+    if len({d.data for d in descs}) != len(descs):
+        raise ValueError("permute: the same field was passed twice")
+    arr = (type(descs[0]) * len(descs))(*descs) if descs else None
+    N.check(N.load().cabana_amd_permute_members(
+        ctx, perm, linked_cell_list.rangeBegin(), linked_cell_list.rangeEnd(),
+        arr, len(descs), N.current_stream()), ctx)

@@ -314,6 +314,18 @@ cabana_amd_status cabana_amd_permute( cabana_amd_context* ctx,
                                       const cabana_amd_particle_field* field,
                                       cabana_amd_stream stream );
 
+/* permute(binning_data, aosoa) (core/src/Cabana_Sort.hpp:549-589): every
+ * member of a particle set at once, field[i] = old field[permutes[i - begin]]
+ * for i in [begin, end) and every member, bitwise.  The members must not
+ * share memory (as the members of one AoSoA do not).  Row-contiguous
+ * members (vector length 1, 4-byte multiples) go through one fused gather
+ * that reads the permutation once; the others are permuted one at a time.
+ * Any dtype. */
+cabana_amd_status cabana_amd_permute_members(
+    cabana_amd_context* ctx, const int64_t* permutes, int64_t begin, int64_t end,
+    const cabana_amd_particle_field* fields, int32_t num_fields,
+    cabana_amd_stream stream );
+
 /* ----------------------------------------------------------------------- */
 /* Kernel timing (HIP events around every launch of this context)           */
 /* ----------------------------------------------------------------------- */

@@ -583,6 +583,23 @@ void permute( const LinkedCellList& lcl, const Field& field )
                    ctx );
 }
 
+// permute(binning_data, aosoa) (core/src/Cabana_Sort.hpp:549-589) over the
+// members given: each is permuted once, the row-contiguous ones in one fused
+// pass.  The members must not share memory.
+template <class Field0, class Field1, class... Fields>
+void permute( const LinkedCellList& lcl, const Field0& f0, const Field1& f1,
+              const Fields&... rest )
+{
+    const cabana_amd_particle_field f[] = { detail::particle_field( f0 ),
+                                            detail::particle_field( f1 ),
+                                            detail::particle_field( rest )... };
+    auto* ctx = detail::context();
+    detail::check( cabana_amd_permute_members( ctx, lcl.permutes(), lcl.rangeBegin(),
+                                               lcl.rangeEnd(), f,
+                                               int32_t( 2 + sizeof...( rest ) ), nullptr ),
+                   ctx );
+}
+
 // ---------------------------------------------------------------------------
 // p2g / g2p (Cabana_Grid_Interpolation.hpp:831-927, 1217-1313)
 // ---------------------------------------------------------------------------

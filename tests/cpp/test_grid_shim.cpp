@@ -374,8 +374,7 @@ static void call_forms_test()
     // binning; owned nodes must match the unbinned result.
     auto lcl = cg::createLinkedCellList( v_pos, { 1.0, 1.0, 1.0 }, { 0.0, 0.0, 0.0 },
                                          { 16.0, 16.0, 16.0 } );
-    cg::permute( *lcl, v_pos );
-    cg::permute( *lcl, v_mass );
+    cg::permute( *lcl, v_pos, v_mass );
     cg::permute( *lcl, v_vel );
     auto gm_c = make_array<double>( lg, 1, n );
     cg::p2g( cg::createScalarValueP2G( v_mass, 1.0 ), v_pos, np, cg::Spline<2>(), *halo, *gm_c,

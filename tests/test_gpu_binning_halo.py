@@ -148,6 +148,56 @@ def test_permute_row_layouts(ca, oracle, begin_end):
     assert np.array_equal(twide.cpu().numpy(), wide_ref)
 
 
+@pytest.mark.parametrize("begin_end", [(0, None), (513, -700)])
+def test_permute_members_fused_and_single(ca, oracle, begin_end):
+    """permute(binning, aosoa) over 19 members: 18 row-contiguous ones (more
+    than one fused pass holds, mixed FP64/FP32/int32/int64 and word counts,
+    one padded-row view) and one AoSoA member (VL = 32), all bitwise equal to
+    indexing by the Serial permutation; the same field twice is rejected."""
+    rng = np.random.default_rng(47)
+    n = 20011
+    begin, end = begin_end[0], (n if begin_end[1] is None else n + begin_end[1])
+    pos = rng.uniform(0, 5.0, size=(n, 3))
+    _, _, perm, _ = oracle.lcl_build(pos, [0.5] * 3, [0.0] * 3, [5.0] * 3,
+                                     begin=begin, end=end)
+    lcl = ca.createLinkedCellList(torch.tensor(pos, device="cuda"), [0.5] * 3, [0.0] * 3,
+                                  [5.0] * 3, begin, end)
+    shapes = [((), np.float64), ((3,), np.float32), ((3, 3), np.float32), ((), np.int32),
+              ((2,), np.int64), ((4,), np.float64), ((5,), np.int32), ((3,), np.float64)]
+    arrays = []
+    for k in range(17):
+        shp, dt = shapes[k % len(shapes)]
+        if np.issubdtype(dt, np.integer):
+            a = rng.integers(-2**31, 2**31 - 1, size=(n,) + shp).astype(dt)
+        else:
+            a = rng.normal(size=(n,) + shp).astype(dt)
+        arrays.append(a)
+    wide = rng.normal(size=(n, 6)).astype(np.float32)
+    tensors = [torch.tensor(a, device="cuda") for a in arrays]
+    twide = torch.tensor(wide, device="cuda")
+    vl = 32
+    nsoa = (n + vl - 1) // vl
+    raw = torch.tensor(rng.normal(size=nsoa * vl * 4), device="cuda")
+    member = ca.AoSoASlice(raw[vl:], n, (3,), vl, vl * 4)
+    m0 = member.to_tensor().cpu().numpy()
+    ca.permute(lcl, *tensors, twide[:, 1:4], member)
+
+    def expect(x):
+        y = x.copy()
+        y[begin:end] = x[perm]
+        return y
+
+    for a, t in zip(arrays, tensors):
+        assert np.array_equal(t.cpu().numpy(), expect(a))
+    wide_ref = wide.copy()
+    wide_ref[:, 1:4] = expect(wide[:, 1:4])
+    assert np.array_equal(twide.cpu().numpy(), wide_ref)
+    assert np.array_equal(member.to_tensor().cpu().numpy(), expect(m0))
+    with pytest.raises(ValueError):
+        ca.permute(lcl, tensors[0], tensors[0])
+    ca.permute(lcl)
+
+
 def test_linked_cell_aosoa_positions_and_permute(ca, oracle):
     """Positions and a tensor member in one AoSoA (VL = 64): build from the
     slice, permute the slice, compare bitwise with the oracle."""
