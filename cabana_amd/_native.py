@@ -75,6 +75,7 @@ class Binning(ctypes.Structure):
         ("begin", c_int64),
         ("end", c_int64),
         ("num_bin", c_int32 * 3),
+        ("serial", ctypes.c_uint64),
     ]
 
 

@@ -42,6 +42,28 @@ struct cabana_amd_context : public Launcher
     void* scratch[5] = { nullptr, nullptr, nullptr, nullptr, nullptr };
     size_t scratch_bytes[5] = { 0, 0, 0, 0, 0 };
 
+    // Brick part tables of binnings that carry a build serial, each in its
+    // own allocation (the general scratch slots move when they grow); the
+    // least recently used one is rebuilt when a new key arrives.
+    struct PartTable
+    {
+        const void* counts = nullptr;
+        const void* offsets = nullptr;
+        int64_t begin = 0;
+        int64_t end = 0;
+        uint64_t serial = 0; // 0: empty
+        int32_t tile[3] = { 0, 0, 0 };
+        int32_t cap = 0;
+        void* mem = nullptr;
+        size_t bytes = 0;
+        int2* parts = nullptr;
+        int32_t* num_parts = nullptr;
+        uint64_t used = 0;
+    };
+    static constexpr int PART_TABLES = 4;
+    PartTable tables[PART_TABLES];
+    uint64_t table_clock = 0;
+
     // Kernel timing.
     bool profiling = false;
     std::vector<Pending> pending;
@@ -667,14 +689,38 @@ bool make_tiles( TileKind kind, const MeshDev& m, int order, bool fp64,
 }
 
 // Builds the part table of a tiling in the context scratch and sets the
-// launch grid.
+// launch grid.  A binning with a build serial keeps its table in the
+// context's table cache: a later call with the same binning and tile shape
+// reuses it without launching anything.
 cabana_amd_status make_parts( cabana_amd_context* ctx, const MeshDev& m,
                               const cabana_amd_binning* bins, TileArgs& t,
                               hipStream_t s )
 {
+    using PartTable = cabana_amd_context::PartTable;
     const int64_t nbricks = (int64_t)t.ntile[0] * t.ntile[1] * t.ntile[2];
     const int64_t nbin = bins->end - bins->begin;
     const int64_t max_parts = nbricks + ( nbin + t.cap - 1 ) / t.cap;
+    const int64_t grid = ( max_parts + 7 ) / 8 * 8;
+    t.grid = (int32_t)( grid < max_tile_groups() ? grid : max_tile_groups() );
+    PartTable* pt = nullptr;
+    if ( bins->serial != 0 )
+    {
+        for ( auto& e : ctx->tables )
+            if ( e.serial == bins->serial && e.counts == bins->counts &&
+                 e.offsets == bins->offsets && e.begin == bins->begin && e.end == bins->end &&
+                 e.tile[0] == t.tile[0] && e.tile[1] == t.tile[1] && e.tile[2] == t.tile[2] &&
+                 e.cap == t.cap )
+            {
+                e.used = ++ctx->table_clock;
+                t.parts = e.parts;
+                t.num_parts = e.num_parts;
+                return CABANA_AMD_OK;
+            }
+        pt = &ctx->tables[0];
+        for ( auto& e : ctx->tables )
+            if ( e.used < pt->used )
+                pt = &e;
+    }
     const int64_t nscan = ( nbricks + 1023 ) / 1024 + 1;
     size_t off = 0;
     auto carve = [&]( size_t bytes ) {
@@ -687,8 +733,27 @@ cabana_amd_status make_parts( cabana_amd_context* ctx, const MeshDev& m,
     const size_t o_bs = carve( sizeof( int64_t ) * nscan );
     const size_t o_pt = carve( sizeof( int2 ) * max_parts );
     const size_t o_n = carve( sizeof( int32_t ) );
-    CA_HIP( ctx, ctx->ensure( SLOT_TILES, off ), "scratch" );
-    char* base = static_cast<char*>( ctx->scratch[SLOT_TILES] );
+    char* base = nullptr;
+    if ( pt )
+    {
+        pt->serial = 0; // empty until rebuilt below
+        if ( pt->bytes < off )
+        {
+            if ( pt->mem )
+                hipFree( pt->mem );
+            pt->mem = nullptr;
+            pt->bytes = 0;
+            const size_t want = off + off / 4 + 256;
+            CA_HIP( ctx, hipMalloc( &pt->mem, want ), "part table" );
+            pt->bytes = want;
+        }
+        base = static_cast<char*>( pt->mem );
+    }
+    else
+    {
+        CA_HIP( ctx, ctx->ensure( SLOT_TILES, off ), "scratch" );
+        base = static_cast<char*>( ctx->scratch[SLOT_TILES] );
+    }
     int2* parts = reinterpret_cast<int2*>( base + o_pt );
     int32_t* num_parts = reinterpret_cast<int32_t*>( base + o_n );
     CA_HIP( ctx,
@@ -700,8 +765,20 @@ cabana_amd_status make_parts( cabana_amd_context* ctx, const MeshDev& m,
             "brick parts" );
     t.parts = parts;
     t.num_parts = num_parts;
-    int64_t grid = ( max_parts + 7 ) / 8 * 8;
-    t.grid = (int32_t)( grid < max_tile_groups() ? grid : max_tile_groups() );
+    if ( pt )
+    {
+        pt->counts = bins->counts;
+        pt->offsets = bins->offsets;
+        pt->begin = bins->begin;
+        pt->end = bins->end;
+        for ( int d = 0; d < 3; ++d )
+            pt->tile[d] = t.tile[d];
+        pt->cap = t.cap;
+        pt->parts = parts;
+        pt->num_parts = num_parts;
+        pt->used = ++ctx->table_clock;
+        pt->serial = bins->serial;
+    }
     return CABANA_AMD_OK;
 }
 
@@ -913,6 +990,9 @@ extern "C" cabana_amd_status cabana_amd_context_destroy( cabana_amd_context* ctx
     for ( int i = 0; i < 5; ++i )
         if ( ctx->scratch[i] )
             hipFree( ctx->scratch[i] );
+    for ( auto& pt : ctx->tables )
+        if ( pt.mem )
+            hipFree( pt.mem );
     if ( ctx->skipped )
         hipFree( ctx->skipped );
     delete ctx;

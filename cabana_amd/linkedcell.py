@@ -1,11 +1,18 @@
 """LinkedCellList build and permute over the C ABI
 (core/src/Cabana_LinkedCellList.hpp, core/src/Cabana_Sort.hpp)."""
 import ctypes
+import itertools
 
 import torch
 
 from . import _native as N
 from .grid import particle_descriptor
+
+
+# Build serials (unique in the process; the C++ layer's have the top bit
+# set): the library reuses the brick part tables it derived from one build of
+# counts/offsets while the serial holds.
+_BUILD_SERIAL = itertools.count(1)
 
 
 def _arr3(v):
@@ -46,6 +53,7 @@ class LinkedCellList:
             self.offsets.data_ptr(),
             self.permutes.data_ptr() if self.permutes.numel() else None,
             N.current_stream()), ctx)
+        self._serial = next(_BUILD_SERIAL)
 
     def numBin(self, d):
         return self._nbin[d]
@@ -81,6 +89,7 @@ class LinkedCellList:
         b.end = self._end
         for d in range(3):
             b.num_bin[d] = self._nbin[d]
+        b.serial = self._serial
         return b
 
 

@@ -126,7 +126,12 @@ typedef struct
  * [begin, end), and particles that moved after the binning was built, are
  * interpolated exactly like the rest (through global memory instead of the
  * brick's LDS window).  counts/offsets must still partition [begin, end).
- * NULL means "no ordering is known". */
+ * NULL means "no ordering is known".
+ * serial: 0, or an identifier of this build of counts/offsets that no other
+ * build in the process shares.  Calls that pass the same non-zero serial (and
+ * the same counts, offsets, begin and end) reuse the brick part tables that
+ * an earlier call built from it on this context, in stream order; the owner
+ * of the binning must change the serial whenever counts or offsets change. */
 typedef struct
 {
     const int32_t* counts;  /* per cell */
@@ -134,6 +139,7 @@ typedef struct
     int64_t begin;
     int64_t end;
     int32_t num_bin[3];     /* must equal the mesh's cells per dimension */
+    uint64_t serial;        /* 0: rebuild the part tables in every call */
 } cabana_amd_binning;
 
 typedef struct

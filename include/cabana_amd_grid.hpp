@@ -31,6 +31,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <array>
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -497,6 +498,11 @@ class LinkedCellList
                            static_cast<int32_t*>( _counts ), static_cast<int64_t*>( _offsets ),
                            static_cast<int64_t*>( _permutes ), nullptr ),
                        ctx );
+        // A new serial for every build (unique in the process; the top bit
+        // keeps these apart from the Python layer's serials): the library
+        // reuses part tables derived from one build while the serial holds.
+        static std::atomic<uint64_t> next_serial{ 1 };
+        _serial = ( uint64_t( 1 ) << 63 ) | next_serial.fetch_add( 1 );
     }
 
     int numBin( const int d ) const { return _nbin[d]; }
@@ -539,6 +545,7 @@ class LinkedCellList
         b.end = _end;
         for ( int d = 0; d < 3; ++d )
             b.num_bin[d] = _nbin[d];
+        b.serial = _serial;
         return b;
     }
 
@@ -562,6 +569,7 @@ class LinkedCellList
     int64_t _capacity = 0;
     int64_t _begin = 0;
     int64_t _end = 0;
+    uint64_t _serial = 0;
 };
 
 template <class Positions>

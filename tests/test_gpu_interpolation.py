@@ -450,3 +450,70 @@ def test_dense_cells_multiple_batches(ca, oracle, order, dtype):
     own = c.mesh.owned_slices()
     assert rel_l2(grid.view().cpu().numpy()[own], ref[own]) <= TOL[dtype]
     assert ca.check_domain() == 0
+
+
+class _NoSerial:
+    """The same binning without a build serial (the part tables are rebuilt in
+    every call)."""
+
+    def __init__(self, lcl):
+        self.lcl = lcl
+
+    def binning_descriptor(self):
+        b = self.lcl.binning_descriptor()
+        b.serial = 0
+        return b
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_part_table_reuse_and_rebuild(ca, oracle, order):
+    """A binning with a build serial reuses its brick part tables across
+    calls: repeated tiled P2G calls equal those of the same binning without a
+    serial to FP64 rounding (the grid flush adds through atomics, so the
+    order of the additions varies), tiled G2P calls bitwise.  After the same
+    LinkedCellList is rebuilt in
+    place (same counts/offsets buffers) over a clustered cloud, whose bricks
+    need other part counts, the tiled P2G still matches the oracle."""
+    dtype = torch.float64
+    c = Case(ca, oracle, num_cell=(16, 14, 12), n=60000, seed=91 + order)
+    gm = c.lg.globalGrid().globalMesh()
+    lo = np.array([gm.lowCorner(d) for d in range(3)])
+    hi = np.array([gm.highCorner(d) for d in range(3)])
+    delta = [gm.cellSize(d) for d in range(3)]
+    lcl = ca.createLinkedCellList(c.pts, delta, list(lo), list(hi))
+    pts = c.pts[lcl.permutes].contiguous()
+    mass_np = c.rng.uniform(0.5, 1.5, size=c.n)
+    mass = torch.tensor(mass_np, device="cuda")
+
+    def p2g(bins, points):
+        g = c.array(1, dtype)
+        ca.p2g(ca.createScalarValueP2G(mass, 1.0), points, c.n, ca.Spline(order), c.halo(g), g,
+               bins=bins)
+        return g.view().cpu().numpy()
+
+    def g2p(bins, points, grid):
+        out = torch.zeros(c.n, 3, dtype=dtype, device="cuda")
+        ca.g2p(grid, c.halo(grid), points, c.n, ca.Spline(order),
+               ca.createVectorValueG2P(out, 1.0), bins=bins)
+        return out.cpu().numpy()
+
+    ref = p2g(_NoSerial(lcl), pts)
+    own = c.mesh.owned_slices()
+    for _ in range(3):
+        assert rel_l2(p2g(lcl, pts)[own], ref[own]) <= TOL[dtype]
+    gv = c.array(3, dtype)
+    gv.view().copy_(torch.tensor(c.rng.normal(size=gv.view().shape), dtype=dtype))
+    uref = g2p(_NoSerial(lcl), pts, gv)
+    for _ in range(2):
+        assert np.array_equal(g2p(lcl, pts, gv), uref)
+
+    # In-place rebuild over a cloud clustered around the domain's centre.
+    x2_np = lo + (hi - lo) * np.clip(0.5 + 0.1 * c.rng.normal(size=(c.n, 3)), 0.0, 0.999)
+    lcl.build(torch.tensor(x2_np, device="cuda"), 0, c.n)
+    perm2 = lcl.permutes.cpu().numpy()
+    pts2_np = x2_np[perm2]
+    got = p2g(lcl, torch.tensor(pts2_np, device="cuda"))
+    refg = c.mesh.grid(1)
+    oracle.p2g(c.mesh, order, 0, 1.0, pts2_np, mass_np[:, None], refg)
+    assert rel_l2(got[own], refg[own]) <= TOL[dtype]
+    assert ca.check_domain() == 0
