@@ -60,7 +60,7 @@ struct cabana_amd_context : public Launcher
         int32_t* num_parts = nullptr;
         uint64_t used = 0;
     };
-    static constexpr int PART_TABLES = 4;
+    static constexpr int PART_TABLES = 8;
     PartTable tables[PART_TABLES];
     uint64_t table_clock = 0;
 
