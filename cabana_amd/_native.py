@@ -162,6 +162,10 @@ def library_path():
     return LIB_PATH
 
 
+# CABANA_AMD_ABI_VERSION of include/cabana_amd.h that the structs below mirror.
+ABI_VERSION = 2
+
+
 def load():
     """Loads libcabana_amd.so (raises if it has not been built)."""
     global _lib
@@ -178,6 +182,9 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.cabana_amd_abi_version() != ABI_VERSION:
+        raise RuntimeError("cabana_amd: %s has ABI version %d, these bindings need %d; rebuild it"
+                           % (path, lib.cabana_amd_abi_version(), ABI_VERSION))
     _lib = lib
     return lib
 

@@ -24,7 +24,8 @@
 extern "C" {
 #endif
 
-#define CABANA_AMD_ABI_VERSION 1
+/* 2: cabana_amd_binning gained `serial`. */
+#define CABANA_AMD_ABI_VERSION 2
 
 typedef struct cabana_amd_context cabana_amd_context;
 

@@ -41,7 +41,7 @@ def test_python_bindings_cover_the_header():
 def test_abi_version_and_host_helpers():
     lib = ctypes.CDLL(LIB)
     lib.cabana_amd_abi_version.restype = ctypes.c_int32
-    assert lib.cabana_amd_abi_version() == 1
+    assert lib.cabana_amd_abi_version() == 2
     d3 = ctypes.c_double * 3
     nb = (ctypes.c_int32 * 3)()
     lib.cabana_amd_linked_cell_num_bins.restype = ctypes.c_int32
@@ -52,3 +52,50 @@ def test_abi_version_and_host_helpers():
     assert rc == 1
     lib.cabana_amd_last_error.restype = ctypes.c_char_p
     assert b"Cartesian" in lib.cabana_amd_last_error(None)
+
+
+STRUCTS = {
+    "cabana_amd_mesh": "Mesh",
+    "cabana_amd_particle_field": "ParticleField",
+    "cabana_amd_grid_field": "GridField",
+    "cabana_amd_binning": "Binning",
+    "cabana_amd_p2g_term": "P2GTerm",
+    "cabana_amd_g2p_term": "G2PTerm",
+    "cabana_amd_p2g_adjoint_term": "P2GAdjointTerm",
+    "cabana_amd_g2p_adjoint_term": "G2PAdjointTerm",
+}
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Size and field offsets of every struct of the header, compiled by the
+    C compiler, equal those of the ctypes mirrors."""
+    import shutil
+    import subprocess
+    from cabana_amd import _native
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "cabana_amd.h"',
+             "int main(void) {"]
+    for cname, pname in STRUCTS.items():
+        cls = getattr(_native, pname)
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for field, _ in cls._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));'
+                         % (cname, field, cname, field))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                               text=True).stdout.splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, pname in STRUCTS.items():
+        cls = getattr(_native, pname)
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert got[(cname, field)] == getattr(cls, field).offset, (cname, field)
