@@ -1086,14 +1086,16 @@ extern "C" cabana_amd_status cabana_amd_p2g(
     {
         const bool mpm = num_terms == 3 && T[0].form == FORM_SV &&
                          T[1].form == FORM_VV && T[2].form == FORM_TD;
-        if ( num_terms == 1 || mpm )
+        const bool mm = num_terms == 2 && T[0].form == FORM_SV && T[1].form == FORM_VV;
+        if ( num_terms == 1 || mpm || mm )
         {
             a.nterms = num_terms;
             for ( int i = 0; i < num_terms; ++i )
                 a.t[i] = T[i];
             CA_CHECK( scatter_pass( ctx, order, fp64,
-                                    mpm ? PLAN_MPM_P2G : PLAN_SINGLE, a, n,
-                                    bins, s ) );
+                                    mpm ? PLAN_MPM_P2G
+                                        : ( mm ? PLAN_MASS_MOMENTUM : PLAN_SINGLE ),
+                                    a, n, bins, s ) );
         }
         else
         {

@@ -335,7 +335,8 @@ enum Plan : int32_t
 {
     PLAN_SINGLE = 0,  // one term, any form
     PLAN_MPM_P2G = 1, // (SV, VV, TD): mass, momentum, stress divergence
-    PLAN_MPM_G2P = 2  // (VV, TDT) on one shared grid: velocity + gradient
+    PLAN_MPM_G2P = 2, // (VV, TDT) on one shared grid: velocity + gradient
+    PLAN_MASS_MOMENTUM = 3 // (SV, VV): mass and momentum (P2G only)
 };
 
 // Kernel launchers (return hipSuccess or the launch error), one set per

@@ -358,6 +358,12 @@ hipError_t scatter_atomic_typed( Launcher& L, int plan, const ScatterArgs& a,
                             g, dim3( BLOCK ), 0, s, a );
         e = hipGetLastError();
     }
+    else if ( plan == PLAN_MASS_MOMENTUM && !a.has_xbar )
+    {
+        hipLaunchKernelGGL( ( scatter_atomic_kernel<Real, false, false, FORM_SV, FORM_VV> ),
+                            g, dim3( BLOCK ), 0, s, a );
+        e = hipGetLastError();
+    }
     else if ( plan == PLAN_MPM_G2P )
     {
         if ( a.has_xbar )

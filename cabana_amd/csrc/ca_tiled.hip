@@ -1344,6 +1344,9 @@ hipError_t scatter_cells_launch( int plan, const ScatterArgs& a, const TileArgs&
         return launch_tile_kernel(
             scatter_cells_kernel<Real, CONTIG, false, FORM_SV, FORM_VV, FORM_TD>, a, t, lds, s,
             SBLOCK );
+    if ( plan == PLAN_MASS_MOMENTUM )
+        return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, false, FORM_SV, FORM_VV>,
+                                   a, t, lds, s, SBLOCK );
     if ( plan == PLAN_MPM_G2P )
         return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, true, FORM_VV, FORM_TDT>,
                                    a, t, lds, s, SBLOCK );
@@ -1354,9 +1357,10 @@ template <class Real>
 hipError_t scatter_cells_typed( Launcher& L, int plan, const ScatterArgs& a,
                                 const TileArgs& t, size_t lds, hipStream_t s )
 {
-    const char* name = plan == PLAN_MPM_P2G   ? "p2g_mpm_cells"
-                       : plan == PLAN_MPM_G2P ? "g2p_adjoint_mpm_cells"
-                                              : "scatter_cells";
+    const char* name = plan == PLAN_MPM_P2G         ? "p2g_mpm_cells"
+                       : plan == PLAN_MPM_G2P       ? "g2p_adjoint_mpm_cells"
+                       : plan == PLAN_MASS_MOMENTUM ? "p2g_mass_momentum_cells"
+                                                    : "scatter_cells";
     L.before( name, s );
     const hipError_t e = a.contig ? scatter_cells_launch<Real, true>( plan, a, t, lds, s )
                                   : scatter_cells_launch<Real, false>( plan, a, t, lds, s );

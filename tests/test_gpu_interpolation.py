@@ -517,3 +517,36 @@ def test_part_table_reuse_and_rebuild(ca, oracle, order):
     oracle.p2g(c.mesh, order, 0, 1.0, pts2_np, mass_np[:, None], refg)
     assert rel_l2(got[own], refg[own]) <= TOL[dtype]
     assert ca.check_domain() == 0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_mass_momentum_pair(ca, oracle, order, dtype):
+    """P2G of (ScalarValue, VectorValue) in one call: one fused pass over the
+    cell-sorted particles of a binning of [700, n - 900), the particles
+    outside it through the atomic pass; each grid against the oracle."""
+    c = Case(ca, oracle, num_cell=(14, 12, 11), n=40000, seed=131 + order)
+    gm = c.lg.globalGrid().globalMesh()
+    lo = [gm.lowCorner(d) for d in range(3)]
+    hi = [gm.highCorner(d) for d in range(3)]
+    begin, end = 700, c.n - 900
+    lcl = ca.createLinkedCellList(c.pts, [gm.cellSize(d) for d in range(3)], lo, hi,
+                                  begin, end)
+    perm = np.arange(c.n)
+    perm[begin:end] = lcl.permutes.cpu().numpy()
+    pts_np = c.pts_np[perm]
+    m = c.rng.uniform(0.5, 1.5, size=(c.n, 1))
+    q = m * c.rng.normal(size=(c.n, 3))
+    tm = torch.tensor(m[:, 0], dtype=dtype, device="cuda")
+    tq = torch.tensor(q, dtype=dtype, device="cuda")
+    gmass, gmom = c.array(1, dtype), c.array(3, dtype)
+    ca.p2g([ca.createScalarValueP2G(tm, 0.5), ca.createVectorValueP2G(tq, -2.0)],
+           torch.tensor(pts_np, device="cuda"), c.n, ca.Spline(order), c.halo(gmass),
+           [gmass, gmom], bins=lcl)
+    own = c.mesh.owned_slices()
+    for arr, op, vals, mult, nd in ((gmass, 0, m, 0.5, 1), (gmom, 1, q, -2.0, 3)):
+        ref = c.mesh.grid(nd)
+        oracle.p2g(c.mesh, order, op, mult, pts_np, vals, ref)
+        err = rel_l2(arr.view().cpu().numpy()[own], ref[own])
+        assert err <= TOL[dtype], (op, err)
+    assert ca.check_domain() == 0
