@@ -47,13 +47,15 @@ class LinkedCellList:
                                     device=dev)
         pos = particle_descriptor(positions)
         ctx = N.context()
+        # A new serial before the build: counts/offsets change even when the
+        # build fails part way.
+        self._serial = next(_BUILD_SERIAL)
         N.check(N.load().cabana_amd_linked_cell_build(
             ctx, ctypes.byref(pos), self._begin, self._end, _arr3(self._delta),
             _arr3(self._min), _arr3(self._max), self.counts.data_ptr(),
             self.offsets.data_ptr(),
             self.permutes.data_ptr() if self.permutes.numel() else None,
             N.current_stream()), ctx)
-        self._serial = next(_BUILD_SERIAL)
 
     def numBin(self, d):
         return self._nbin[d]

@@ -491,6 +491,13 @@ class LinkedCellList
         }
         _begin = begin;
         _end = end;
+        // A new serial for every build, taken before it (counts/offsets
+        // change even when the build fails part way); unique in the process,
+        // the top bit keeps these apart from the Python layer's serials.
+        // The library reuses part tables derived from one build while the
+        // serial holds.
+        static std::atomic<uint64_t> next_serial{ 1 };
+        _serial = ( uint64_t( 1 ) << 63 ) | next_serial.fetch_add( 1 );
         auto f = detail::particle_field( positions );
         auto* ctx = detail::context();
         detail::check( cabana_amd_linked_cell_build(
@@ -498,11 +505,6 @@ class LinkedCellList
                            static_cast<int32_t*>( _counts ), static_cast<int64_t*>( _offsets ),
                            static_cast<int64_t*>( _permutes ), nullptr ),
                        ctx );
-        // A new serial for every build (unique in the process; the top bit
-        // keeps these apart from the Python layer's serials): the library
-        // reuses part tables derived from one build while the serial holds.
-        static std::atomic<uint64_t> next_serial{ 1 };
-        _serial = ( uint64_t( 1 ) << 63 ) | next_serial.fetch_add( 1 );
     }
 
     int numBin( const int d ) const { return _nbin[d]; }
