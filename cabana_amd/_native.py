@@ -122,6 +122,7 @@ P = ctypes.POINTER
 # name -> (restype, argtypes); every symbol declared in include/cabana_amd.h.
 SIGNATURES = {
     "cabana_amd_abi_version": (c_int32, []),
+    "cabana_amd_new_binning_serial": (ctypes.c_uint64, []),
     "cabana_amd_context_create": (c_int32, [c_int32, P(c_void_p)]),
     "cabana_amd_context_destroy": (c_int32, [c_void_p]),
     "cabana_amd_last_error": (ctypes.c_char_p, [c_void_p]),
@@ -163,7 +164,7 @@ def library_path():
 
 
 # CABANA_AMD_ABI_VERSION of include/cabana_amd.h that the structs below mirror.
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 def load():

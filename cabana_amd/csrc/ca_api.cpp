@@ -1,11 +1,13 @@
 // C-ABI layer: descriptor validation, plan selection and kernel dispatch.
 #include "ca_internal.hpp"
 
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -15,6 +17,32 @@ namespace
 {
 
 thread_local std::string g_thread_error = "";
+
+// Binning serials issued so far (cabana_amd_new_binning_serial).
+std::atomic<uint64_t> g_serials{ 0 };
+
+// Build generation of every counts/offsets array that
+// cabana_amd_linked_cell_build wrote, across all contexts: a part table
+// records the generations of the arrays it was derived from and is reused
+// only while both are unchanged.
+std::mutex g_gen_mutex;
+std::map<const void*, uint64_t> g_generation;
+uint64_t g_generation_clock = 0;
+
+uint64_t generation_of( const void* p )
+{
+    std::lock_guard<std::mutex> lock( g_gen_mutex );
+    const auto it = g_generation.find( p );
+    return it == g_generation.end() ? 0 : it->second;
+}
+
+void bump_generation( const void* a, const void* b )
+{
+    std::lock_guard<std::mutex> lock( g_gen_mutex );
+    const uint64_t g = ++g_generation_clock;
+    g_generation[a] = g;
+    g_generation[b] = g;
+}
 
 struct Pending
 {
@@ -52,6 +80,8 @@ struct cabana_amd_context : public Launcher
         int64_t begin = 0;
         int64_t end = 0;
         uint64_t serial = 0; // 0: empty
+        uint64_t gen_counts = 0;  // build generations of counts/offsets
+        uint64_t gen_offsets = 0; // when the table was made
         int32_t tile[3] = { 0, 0, 0 };
         int32_t cap = 0;
         void* mem = nullptr;
@@ -703,11 +733,19 @@ cabana_amd_status make_parts( cabana_amd_context* ctx, const MeshDev& m,
     const int64_t grid = ( max_parts + 7 ) / 8 * 8;
     t.grid = (int32_t)( grid < max_tile_groups() ? grid : max_tile_groups() );
     PartTable* pt = nullptr;
+    uint64_t gen_c = 0, gen_o = 0;
     if ( bins->serial != 0 )
     {
+        if ( bins->serial > g_serials.load() )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "binning serial was not issued by cabana_amd_new_binning_serial "
+                         "(zero-initialise cabana_amd_binning)" );
+        gen_c = generation_of( bins->counts );
+        gen_o = generation_of( bins->offsets );
         for ( auto& e : ctx->tables )
             if ( e.serial == bins->serial && e.counts == bins->counts &&
                  e.offsets == bins->offsets && e.begin == bins->begin && e.end == bins->end &&
+                 e.gen_counts == gen_c && e.gen_offsets == gen_o &&
                  e.tile[0] == t.tile[0] && e.tile[1] == t.tile[1] && e.tile[2] == t.tile[2] &&
                  e.cap == t.cap )
             {
@@ -769,6 +807,8 @@ cabana_amd_status make_parts( cabana_amd_context* ctx, const MeshDev& m,
     {
         pt->counts = bins->counts;
         pt->offsets = bins->offsets;
+        pt->gen_counts = gen_c;
+        pt->gen_offsets = gen_o;
         pt->begin = bins->begin;
         pt->end = bins->end;
         for ( int d = 0; d < 3; ++d )
@@ -941,6 +981,8 @@ cabana_amd_status validate_bins( cabana_amd_context* ctx,
 // ---------------------------------------------------------------------------
 
 extern "C" int32_t cabana_amd_abi_version( void ) { return CABANA_AMD_ABI_VERSION; }
+
+extern "C" uint64_t cabana_amd_new_binning_serial( void ) { return ++g_serials; }
 
 extern "C" cabana_amd_status cabana_amd_context_create( int32_t device,
                                                         cabana_amd_context** out )
@@ -1586,6 +1628,9 @@ extern "C" cabana_amd_status cabana_amd_linked_cell_build(
                      "more than 2^31 - 2 bins" );
     hipStream_t s = static_cast<hipStream_t>( stream );
     const int64_t np = end - begin;
+    // Part tables derived from the earlier contents of counts/offsets are
+    // retired now, before anything is written.
+    bump_generation( counts, offsets );
     // Keys are 0..ncell-1 and ncell for particles outside the grid.
     int key_bits = 0;
     while ( ( ncell >> key_bits ) != 0 )

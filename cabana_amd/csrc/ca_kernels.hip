@@ -344,7 +344,9 @@ hipError_t scatter_atomic_typed( Launcher& L, int plan, const ScatterArgs& a,
     const char* name =
         a.has_xbar
             ? ( plan == PLAN_MPM_G2P ? "g2p_adjoint_mpm_atomic" : "g2p_adjoint_atomic" )
-            : ( plan == PLAN_MPM_P2G ? "p2g_mpm_atomic" : "p2g_atomic" );
+            : ( plan == PLAN_MPM_P2G         ? "p2g_mpm_atomic"
+                : plan == PLAN_MASS_MOMENTUM ? "p2g_mass_momentum_atomic"
+                                             : "p2g_atomic" );
     L.before( name, s );
     if ( plan == PLAN_SINGLE )
     {

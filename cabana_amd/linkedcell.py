@@ -1,18 +1,11 @@
 """LinkedCellList build and permute over the C ABI
 (core/src/Cabana_LinkedCellList.hpp, core/src/Cabana_Sort.hpp)."""
 import ctypes
-import itertools
 
 import torch
 
 from . import _native as N
 from .grid import particle_descriptor
-
-
-# Build serials (unique in the process; the C++ layer's have the top bit
-# set): the library reuses the brick part tables it derived from one build of
-# counts/offsets while the serial holds.
-_BUILD_SERIAL = itertools.count(1)
 
 
 def _arr3(v):
@@ -47,9 +40,11 @@ class LinkedCellList:
                                     device=dev)
         pos = particle_descriptor(positions)
         ctx = N.context()
-        # A new serial before the build: counts/offsets change even when the
-        # build fails part way.
-        self._serial = next(_BUILD_SERIAL)
+        # A new serial (issued by the library, unique in the process) before
+        # the build: counts/offsets change even when the build fails part way.
+        # The library reuses the brick part tables it derived from one build
+        # while the serial holds.
+        self._serial = N.load().cabana_amd_new_binning_serial()
         N.check(N.load().cabana_amd_linked_cell_build(
             ctx, ctypes.byref(pos), self._begin, self._end, _arr3(self._delta),
             _arr3(self._min), _arr3(self._max), self.counts.data_ptr(),

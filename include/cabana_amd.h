@@ -24,8 +24,9 @@
 extern "C" {
 #endif
 
-/* 2: cabana_amd_binning gained `serial`. */
-#define CABANA_AMD_ABI_VERSION 2
+/* 2: cabana_amd_binning gained `serial`.
+ * 3: serials are issued by cabana_amd_new_binning_serial. */
+#define CABANA_AMD_ABI_VERSION 3
 
 typedef struct cabana_amd_context cabana_amd_context;
 
@@ -128,11 +129,16 @@ typedef struct
  * interpolated exactly like the rest (through global memory instead of the
  * brick's LDS window).  counts/offsets must still partition [begin, end).
  * NULL means "no ordering is known".
- * serial: 0, or an identifier of this build of counts/offsets that no other
- * build in the process shares.  Calls that pass the same non-zero serial (and
+ * serial: 0, or a value returned by cabana_amd_new_binning_serial for this
+ * build of counts/offsets.  Calls that pass the same non-zero serial (and
  * the same counts, offsets, begin and end) reuse the brick part tables that
  * an earlier call built from it on this context, in stream order; the owner
- * of the binning must change the serial whenever counts or offsets change. */
+ * of the binning takes a new serial whenever counts or offsets change.  A
+ * cabana_amd_linked_cell_build into counts/offsets retires every table built
+ * from their earlier contents, whatever serial the caller passes later, and a
+ * serial the library never issued is rejected (CABANA_AMD_ERROR_INVALID_ARGUMENT).
+ * Zero-initialise the struct (= {0} / {}) so that a caller that does not set
+ * the field passes 0. */
 typedef struct
 {
     const int32_t* counts;  /* per cell */
@@ -174,6 +180,11 @@ cabana_amd_status cabana_amd_context_destroy( cabana_amd_context* ctx );
 const char* cabana_amd_last_error( const cabana_amd_context* ctx );
 
 int32_t cabana_amd_abi_version( void );
+
+/* A binning serial no earlier call returned in this process (never 0;
+ * thread safe).  No reference counterpart: it keys the library's reuse of the
+ * brick part tables it derives from one LinkedCellList build. */
+uint64_t cabana_amd_new_binning_serial( void );
 
 /* Waits for the stream, then reports and clears the count of particles that
  * were skipped because their spline stencil left the ghosted grid (Cabana

@@ -492,12 +492,10 @@ class LinkedCellList
         _begin = begin;
         _end = end;
         // A new serial for every build, taken before it (counts/offsets
-        // change even when the build fails part way); unique in the process,
-        // the top bit keeps these apart from the Python layer's serials.
-        // The library reuses part tables derived from one build while the
-        // serial holds.
-        static std::atomic<uint64_t> next_serial{ 1 };
-        _serial = ( uint64_t( 1 ) << 63 ) | next_serial.fetch_add( 1 );
+        // change even when the build fails part way) from the library, so it
+        // is unique in the process whatever the Positions type.  The library
+        // reuses part tables derived from one build while the serial holds.
+        _serial = cabana_amd_new_binning_serial();
         auto f = detail::particle_field( positions );
         auto* ctx = detail::context();
         detail::check( cabana_amd_linked_cell_build(

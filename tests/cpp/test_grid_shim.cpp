@@ -451,10 +451,101 @@ static void call_forms_test()
                  e2, ( lhs - rhs ) / std::fabs( rhs ) );
 }
 
+// One LinkedCellList rebuilt in place with another Positions type (a View,
+// then an AoSoA slice) over a clustered cloud: the binned P2G after the
+// rebuild must equal the unbinned one.  The library issues the build serials
+// and retires the part tables of the earlier build, so the dense bricks of
+// the second cloud are cut into enough parts and no particle is dropped.
+static void rebuild_test()
+{
+    const int cells = 16, h = 1;
+    int n[3] = { cells, cells, cells };
+    GlobalMesh gm{ { 0.0, 0.0, 0.0 }, { 16.0, 16.0, 16.0 }, 1.0 };
+    auto lg = std::make_shared<LocalGrid>( LocalGrid{ GlobalGrid{ gm }, h } );
+    const std::size_t np = 30000;
+    constexpr int VL = 16;
+    std::mt19937_64 rng( 99 );
+    std::uniform_real_distribution<double> U( 0.0, 16.0 );
+    std::normal_distribution<double> Nd( 0.0, 1.0 );
+
+    // First cloud: uniform, in Views; a binned P2G caches its part table.
+    std::vector<double> pos1( np * 3 ), mass1( np );
+    for ( std::size_t p = 0; p < np; ++p )
+    {
+        for ( int d = 0; d < 3; ++d )
+            pos1[3 * p + d] = U( rng );
+        mass1[p] = 0.5 + U( rng ) / 16.0;
+    }
+    DeviceView<double, 2> v_pos( { np, 3 } );
+    DeviceView<double, 1> v_mass( { np } );
+    v_pos.from_host( pos1 );
+    v_mass.from_host( mass1 );
+    auto lcl = cg::createLinkedCellList( v_pos, { 1.0, 1.0, 1.0 }, { 0.0, 0.0, 0.0 },
+                                         { 16.0, 16.0, 16.0 } );
+    cg::permute( *lcl, v_pos, v_mass );
+    auto halo_grid = make_array<double>( lg, 1, n );
+    auto halo = cg::createHalo( cg::NodeHaloPattern<3>(), h, *halo_grid );
+    auto g1 = make_array<double>( lg, 1, n );
+    cg::p2g( cg::createScalarValueP2G( v_mass, 1.0 ), v_pos, np, cg::Spline<2>(), *halo, *g1,
+             lcl.get() );
+
+    // Second cloud: one tight cluster, in an AoSoA; the same list rebuilt.
+    const std::size_t nsoa = ( np + VL - 1 ) / VL;
+    std::vector<double> aosoa( nsoa * 7 * VL, 0.0 );
+    for ( std::size_t p = 0; p < np; ++p )
+    {
+        const std::size_t s = p / VL, a = p % VL;
+        for ( int d = 0; d < 3; ++d )
+        {
+            double x = 8.0 + 0.7 * Nd( rng );
+            x = x < 0.0 ? 0.0 : ( x > 15.999 ? 15.999 : x );
+            aosoa[s * 7 * VL + d * VL + a] = x;
+        }
+        aosoa[s * 7 * VL + 3 * VL + a] = 0.5 + U( rng ) / 16.0;
+    }
+    double* dev = nullptr;
+    HIP_OK( hipMalloc( &dev, aosoa.size() * sizeof( double ) ) );
+    HIP_OK( hipMemcpy( dev, aosoa.data(), aosoa.size() * sizeof( double ),
+                       hipMemcpyHostToDevice ) );
+    std::shared_ptr<double> mem( dev, []( double* q ) { (void)hipFree( q ); } );
+    AoSoASlice<VL, 3> s_pos{ mem, 0, np };
+    AoSoASlice<VL, 1> s_mass{ mem, 3 * VL, np };
+    auto unbinned = make_array<double>( lg, 1, n );
+    cg::p2g( cg::createScalarValueP2G( s_mass, 1.0 ), s_pos, np, cg::Spline<2>(), *halo,
+             *unbinned );
+    lcl->build( s_pos, 0, np );
+    cg::permute( *lcl, s_pos );
+    cg::permute( *lcl, s_mass );
+    auto binned = make_array<double>( lg, 1, n );
+    cg::p2g( cg::createScalarValueP2G( s_mass, 1.0 ), s_pos, np, cg::Spline<2>(), *halo,
+             *binned, lcl.get() );
+    auto a = unbinned->v.to_host(), b = binned->v.to_host();
+    std::vector<double> ao, bo;
+    const auto& v = unbinned->v;
+    double total = 0.0;
+    for ( int i = h; i < h + cells; ++i )
+        for ( int j = h; j < h + cells; ++j )
+            for ( int k = h; k < h + cells; ++k )
+            {
+                ao.push_back( a[i * v.str[0] + j * v.str[1] + k * v.str[2]] );
+                bo.push_back( b[i * v.str[0] + j * v.str[1] + k * v.str[2]] );
+                total += bo.back();
+            }
+    double msum = 0.0;
+    for ( std::size_t p = 0; p < np; ++p )
+        msum += aosoa[( p / VL ) * 7 * VL + 3 * VL + p % VL];
+    const double e = rel_diff( bo, ao );
+    EXPECT_NEAR( e, 0.0, 1e-13 );
+    EXPECT_NEAR( total / msum, 1.0, 1e-12 );
+    std::printf( "rebuild with another positions type: binned vs unbinned %.2e, mass %.15f\n",
+                 e, total / msum );
+}
+
 int main()
 {
     reference_interpolation_test();
     call_forms_test();
+    rebuild_test();
     HIP_OK( hipDeviceSynchronize() );
     std::printf( "%d checks, %d failures\n", g_checks, g_failures );
     if ( g_failures )

@@ -41,7 +41,7 @@ def test_python_bindings_cover_the_header():
 def test_abi_version_and_host_helpers():
     lib = ctypes.CDLL(LIB)
     lib.cabana_amd_abi_version.restype = ctypes.c_int32
-    assert lib.cabana_amd_abi_version() == 2
+    assert lib.cabana_amd_abi_version() == 3
     d3 = ctypes.c_double * 3
     nb = (ctypes.c_int32 * 3)()
     lib.cabana_amd_linked_cell_num_bins.restype = ctypes.c_int32
@@ -99,3 +99,14 @@ def test_ctypes_structs_match_the_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for field, _ in cls._fields_:
             assert got[(cname, field)] == getattr(cls, field).offset, (cname, field)
+
+
+def test_binning_serials_are_issued_by_the_library():
+    """Serials come from one library-wide counter: never 0, never repeated,
+    and independent of the caller (Python here, the C++ layer elsewhere)."""
+    from cabana_amd import _native
+    lib = _native.load()
+    got = [lib.cabana_amd_new_binning_serial() for _ in range(100)]
+    assert all(s > 0 for s in got)
+    assert len(set(got)) == len(got)
+    assert got == sorted(got)

@@ -11,7 +11,7 @@ from test_gpu_interpolation import (G2P_FACTORY, G2P_SHAPE, P2G_FACTORY, P2G_SHA
 pytestmark = pytest.mark.gpu
 
 TOL = {torch.float64: 1e-12, torch.float32: 1e-5}
-XTOL = {torch.float64: 1e-11, torch.float32: 1e-4}
+XTOL = {torch.float64: 1e-11, torch.float32: 1e-5}
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])

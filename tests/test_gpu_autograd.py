@@ -174,7 +174,7 @@ def test_differentiation_fp32_mixed_vs_fp64(ca):
     ev = (torch.linalg.norm(grads[torch.float32][1] - grads[torch.float64][1]) /
           torch.linalg.norm(grads[torch.float64][1])).item()
     print("FP32-mixed vs FP64 gradient rel-L2: positions %.3e, values %.3e" % (ex, ev))
-    assert ex <= 1e-4 and ev <= 1e-5
+    assert ex <= 1e-5 and ev <= 1e-5
 
 
 def _loop_case(ca, n=3000, cells=12, seed=5):
@@ -207,9 +207,21 @@ def test_checkpointed_loop_matches_plain(ca):
         assert torch.allclose(a, b, rtol=1e-9, atol=1e-11)
 
 
+def _knot_distance(x):
+    """Distance of every coordinate to the nearest knot of the quadratic
+    spline on the unit-cell box (the half-integers: node 0 of the ghosted
+    grid is at -1 and Spline<2> adds 0.5), where the second derivative of
+    the weights jumps."""
+    return (torch.remainder(x - 0.5, 1.0) - 0.5).abs().neg().add(0.5)
+
+
 def test_loop_gradient_fd(ca):
     """Whole-loop gradient (re-binning gather, P2G, grid update, G2P,
-    advection; two steps, checkpointed) against central differences."""
+    advection; two steps, checkpointed) against central differences.  The
+    spline weights are C1 with second derivatives that jump at the knots, so
+    the perturbed particles are chosen at least 1e-3 from every knot at both
+    steps: a central difference across a jump in f'' is off by O(h), which
+    would hide a real error of that size."""
     mpm, box, x0, v0, m = _loop_case(ca, n=2000, seed=11)
     x = x0.clone().requires_grad_()
     v = v0.clone().requires_grad_()
@@ -219,10 +231,16 @@ def test_loop_gradient_fd(ca):
     h = 1e-6
     worst = 0.0
     with torch.no_grad():
+        x1, _, _, _ = mpm.run_loop(box, x0, v0, m, num_steps=1, checkpoint_every=0)
+        clear = ((_knot_distance(x0).min(dim=1).values > 1e-3) &
+                 (_knot_distance(x1).min(dim=1).values > 1e-3)).cpu().numpy()
+        candidates = np.flatnonzero(clear)
+        assert candidates.size >= 12
+
         def loss_at(xx, vv):
             xs, vs, _, _ = mpm.run_loop(box, xx, vv, m, num_steps=2, checkpoint_every=0)
             return _loop_loss(xs, vs).item()
-        for p in rng.choice(x0.shape[0], size=12, replace=False):
+        for p in rng.choice(candidates, size=12, replace=False):
             for d in range(3):
                 for which, grad in (("x", x.grad), ("v", v.grad)):
                     xp, xm = x0.clone(), x0.clone()
@@ -238,4 +256,4 @@ def test_loop_gradient_fd(ca):
                     err = abs(ad - fd) / max(1.0, abs(fd))
                     worst = max(worst, err)
     print("loop gradient vs FD: worst relative error %.3e" % worst)
-    assert worst <= 1e-5
+    assert worst <= 1e-6

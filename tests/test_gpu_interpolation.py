@@ -519,12 +519,15 @@ def test_part_table_reuse_and_rebuild(ca, oracle, order):
     assert ca.check_domain() == 0
 
 
+@pytest.mark.parametrize("vl", [None, 16, 32, 64])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("order", [1, 2, 3])
-def test_mass_momentum_pair(ca, oracle, order, dtype):
+def test_mass_momentum_pair(ca, oracle, order, dtype, vl):
     """P2G of (ScalarValue, VectorValue) in one call: one fused pass over the
     cell-sorted particles of a binning of [700, n - 900), the particles
-    outside it through the atomic pass; each grid against the oracle."""
+    outside it through the atomic pass; each grid against the oracle.  The
+    attributes are contiguous tensors (vl None) or members of one AoSoA of
+    vector length vl (the interleaved instantiation of the fused pass)."""
     c = Case(ca, oracle, num_cell=(14, 12, 11), n=40000, seed=131 + order)
     gm = c.lg.globalGrid().globalMesh()
     lo = [gm.lowCorner(d) for d in range(3)]
@@ -539,6 +542,15 @@ def test_mass_momentum_pair(ca, oracle, order, dtype):
     q = m * c.rng.normal(size=(c.n, 3))
     tm = torch.tensor(m[:, 0], dtype=dtype, device="cuda")
     tq = torch.tensor(q, dtype=dtype, device="cuda")
+    if vl is not None:
+        nsoa = (c.n + vl - 1) // vl
+        stride = vl * (1 + 3)
+        raw = torch.zeros(nsoa * stride, dtype=dtype, device="cuda")
+        sm = ca.AoSoASlice(raw[0:], c.n, (), vl, stride)
+        sq = ca.AoSoASlice(raw[vl:], c.n, (3,), vl, stride)
+        sm.from_tensor(tm)
+        sq.from_tensor(tq)
+        tm, tq = sm, sq
     gmass, gmom = c.array(1, dtype), c.array(3, dtype)
     ca.p2g([ca.createScalarValueP2G(tm, 0.5), ca.createVectorValueP2G(tq, -2.0)],
            torch.tensor(pts_np, device="cuda"), c.n, ca.Spline(order), c.halo(gmass),
