@@ -59,14 +59,19 @@ def sort_by_cell(box, x, *fields):
     return (lcl, permute_tensor(x, perm)) + tuple(permute_tensor(f, perm) for f in fields)
 
 
-def explicit_step(box, x, v, m, dt=0.1, body_force=(0.0, 0.0, -1e-3), resort=True):
-    """One transfer step; returns (x, v, m, velocity_gradient)."""
+def explicit_step(box, x, v, m, dt=0.1, body_force=(0.0, 0.0, -1e-3), resort=True,
+                  record=None):
+    """One transfer step; returns (x, v, m, velocity_gradient).  When
+    `record` is a list, the step appends its P2G grids (owned mass and
+    momentum nodes, detached) to it."""
     bins = None
     if resort:
         bins, x, v, m = sort_by_cell(box, x, v, m)
     lg = box.local_grid
     mass, mom = AG.p2g(x, [m, m[:, None] * v], lg, ["ScalarValue", "VectorValue"],
                        order=2, bins=bins)
+    if record is not None:
+        record.append((mass.detach(), mom.detach()))
     has_mass = mass > 1e-12
     f = torch.tensor(body_force, dtype=mom.dtype, device=mom.device)
     u = torch.where(has_mass, mom / torch.where(has_mass, mass, torch.ones_like(mass)) + dt * f,

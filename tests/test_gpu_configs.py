@@ -1,0 +1,305 @@
+"""The benchmark configurations at their stated shapes, against the oracle.
+
+- 32^3 periodic cells x 8 ppc (262,144 particles), ScalarValueP2G of mass then
+  VectorValueG2P of the node field u_d = sin(2 pi x_d / L): FP64 through the
+  drop-in (unbinned) calls at 1e-12 relative L2 for linear splines, and the
+  FP32-mixed form with linear, quadratic and cubic splines at 1e-5 per field.
+- 128^3 periodic cells x 16 ppc (33,554,432 particles), quadratic, FP64: the
+  fused P2G (mass, momentum, stress divergence) over every particle against
+  the oracle's OpenMP P2G, and the fused G2P (velocity, velocity gradient)
+  against the oracle on a sample of particles (G2P is per particle, so a
+  sample checks it exactly).
+- 256^3 periodic cells with 2^26 particles in 32 Gaussian clusters over a 10 %
+  uniform background, FP32-mixed, quadratic: mass conservation at every step
+  of a 10-step MPM loop, one step's fields and gradients against the FP64
+  path, and checkpointed against plain gradients over 4 steps; and a reduced
+  clustered case (32^3 cells, hundreds of particles in the densest cells)
+  against the oracle.
+
+Semantics follow grid/unit_test/tstInterpolation3d.hpp:118-250 (p2g adds into
+a zeroed array and folds the ghosts; g2p fills the ghosts and adds into the
+particle field).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_interpolation import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh(ca, oracle, cells):
+    L = float(cells)
+    gm = ca.createUniformGlobalMesh([0.0] * 3, [L] * 3, 1.0)
+    lg = ca.createLocalGrid(ca.createGlobalGrid(gm, [True] * 3), 1)
+    return lg, oracle.OMesh([0.0] * 3, [L] * 3, 1.0, halo=1)
+
+
+def _sin_field(cells, shape):
+    """u_d = sin(2 pi x_d / L) at every ghosted node (node i at x = i - 1)."""
+    x = np.arange(shape[0]) - 1.0
+    s = np.sin(2.0 * math.pi * x / cells)
+    u = np.zeros(shape)
+    u[..., 0] = s[:, None, None]
+    u[..., 1] = s[None, :, None]
+    u[..., 2] = s[None, None, :]
+    return u
+
+
+def _config1_inputs(cells=32, ppc=8, seed=20260101):
+    rng = np.random.default_rng(seed)
+    n = cells ** 3 * ppc
+    pts = rng.uniform(0.0, float(cells), size=(n, 3))
+    mass = rng.uniform(0.5, 1.5, size=n)
+    return n, pts, mass
+
+
+@pytest.mark.parametrize("dtype,order", [(torch.float64, 1), (torch.float32, 1),
+                                         (torch.float32, 2), (torch.float32, 3)])
+def test_config1_and_3_full_shape(ca, oracle, dtype, order):
+    """32^3 x 8 ppc: ScalarValueP2G(mass) then VectorValueG2P of the sin
+    field, both through the unbinned drop-in calls and through a binning."""
+    cells = 32
+    n, pts_np, mass_np = _config1_inputs(cells)
+    print("config 1/3: %d particles, order %d, %s" % (n, order, dtype), flush=True)
+    lg, mesh = _mesh(ca, oracle, cells)
+    own = mesh.owned_slices()
+    rho_ref = mesh.grid(1)
+    oracle.p2g(mesh, order, 0, 1.0, pts_np, mass_np[:, None], rho_ref)
+    u_np = _sin_field(cells, mesh.num_node + (3,))
+    vel_ref = np.zeros((n, 3))
+    oracle.g2p(mesh, order, 1, 1.0, u_np.copy(), pts_np, vel_ref)
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    pts = torch.tensor(pts_np, device="cuda")
+    mass = torch.tensor(mass_np, dtype=dtype, device="cuda")
+    lcl = ca.createLinkedCellList(pts, [1.0] * 3, [0.0] * 3, [float(cells)] * 3)
+    perm = lcl.permutes
+    for bins in (None, lcl):
+        p = pts if bins is None else pts[perm].contiguous()
+        mv = mass if bins is None else mass[perm].contiguous()
+        rho = ca.createArray("rho", ca.createArrayLayout(lg, 1, ca.Node()), dtype=dtype)
+        u = ca.createArray("u", ca.createArrayLayout(lg, 3, ca.Node()), dtype=dtype)
+        u.view().copy_(torch.tensor(u_np, dtype=dtype))
+        halo = ca.createHalo(ca.NodeHaloPattern(), 1, rho)
+        ca.p2g(ca.createScalarValueP2G(mv, 1.0), p, n, ca.Spline(order), halo, rho, bins=bins)
+        vel = torch.zeros(n, 3, dtype=dtype, device="cuda")
+        ca.g2p(u, halo, p, n, ca.Spline(order), ca.createVectorValueG2P(vel, 1.0), bins=bins)
+        got_v = vel.double().cpu().numpy()
+        if bins is not None:
+            inv = np.empty(n, dtype=np.int64)
+            inv[perm.cpu().numpy()] = np.arange(n)
+            got_v = got_v[inv]
+        e_rho = rel_l2(rho.view().double().cpu().numpy()[own], rho_ref[own])
+        e_vel = rel_l2(got_v, vel_ref)
+        print("  %s: rel-L2 mass %.3e, velocity %.3e" % ("unbinned" if bins is None else
+                                                        "binned", e_rho, e_vel))
+        assert e_rho <= tol and e_vel <= tol
+        # Mass is conserved by the fold: the owned nodes hold every particle's mass.
+        tot = rho.view().double()[tuple(own)].sum().item()
+        assert abs(tot / mass_np.sum() - 1.0) <= (1e-13 if dtype == torch.float64 else 1e-6)
+    assert ca.check_domain() == 0
+
+
+def _symmetric_stress(rng, n):
+    s6 = rng.normal(size=(n, 6))
+    return np.stack([s6[:, 0], s6[:, 3], s6[:, 4], s6[:, 3], s6[:, 1], s6[:, 5],
+                     s6[:, 4], s6[:, 5], s6[:, 2]], axis=1)
+
+
+def test_config2_full_shape_fp64(ca, oracle):
+    """128^3 x 16 ppc, quadratic, FP64: fused P2G {mass, momentum, stress
+    divergence (-1)} over all 33.5 M particles against the oracle's OpenMP
+    P2G; fused G2P {velocity, velocity gradient} of the sin field against the
+    oracle on 1 M sampled particles."""
+    import os
+    cells, ppc = 128, 16
+    n = cells ** 3 * ppc
+    rng = np.random.default_rng(20260102)
+    print("config 2: generating %d particles" % n, flush=True)
+    pts_np = rng.uniform(0.0, float(cells), size=(n, 3))
+    m_np = rng.uniform(0.5, 1.5, size=n)
+    q_np = m_np[:, None] * rng.normal(size=(n, 3))
+    sig_np = _symmetric_stress(rng, n)
+    lg, mesh = _mesh(ca, oracle, cells)
+    dt = torch.float64
+    pts = torch.tensor(pts_np, device="cuda")
+    lcl = ca.createLinkedCellList(pts, [1.0] * 3, [0.0] * 3, [float(cells)] * 3)
+    perm = lcl.permutes
+    perm_np = perm.cpu().numpy()
+    pts_s = pts[perm].contiguous()
+    del pts
+    m = torch.tensor(m_np, device="cuda")[perm].contiguous()
+    q = torch.tensor(q_np, device="cuda")[perm].contiguous()
+    sig = torch.tensor(sig_np, device="cuda")[perm].contiguous().reshape(n, 3, 3)
+    arr = lambda d: ca.createArray("g", ca.createArrayLayout(lg, d, ca.Node()), dtype=dt)  # noqa
+    gmass, gmom, gstress, u = arr(1), arr(3), arr(3), arr(3)
+    halo = ca.createHalo(ca.NodeHaloPattern(), 1, gmass)
+    ca.p2g([ca.createScalarValueP2G(m, 1.0), ca.createVectorValueP2G(q, 1.0),
+            ca.createTensorDivergenceP2G(sig, -1.0)], pts_s, n, ca.Spline(2), halo,
+           [gmass, gmom, gstress], bins=lcl)
+    u_np = _sin_field(cells, mesh.num_node + (3,))
+    u.view().copy_(torch.tensor(u_np))
+    vel = torch.zeros(n, 3, dtype=dt, device="cuda")
+    grad = torch.zeros(n, 3, 3, dtype=dt, device="cuda")
+    ca.g2p([u, u], halo, pts_s, n, ca.Spline(2),
+           [ca.createVectorValueG2P(vel, 1.0), ca.createVectorGradientG2P(grad, 1.0)],
+           bins=lcl)
+    torch.cuda.synchronize()
+    assert ca.check_domain() == 0
+    own = mesh.owned_slices()
+    threads = min(16, len(os.sched_getaffinity(0)))
+    print("config 2: oracle P2G over all particles on %d threads" % threads, flush=True)
+    for name, arr_, op, vals, mult, nd in (("mass", gmass, 0, m_np[:, None], 1.0, 1),
+                                           ("momentum", gmom, 1, q_np, 1.0, 3),
+                                           ("stress divergence", gstress, 4, sig_np, -1.0, 3)):
+        ref = mesh.grid(nd)
+        oracle.p2g(mesh, 2, op, mult, pts_np, vals, ref, threads=threads)
+        err = rel_l2(arr_.view().cpu().numpy()[own], ref[own])
+        print("  P2G %s: rel-L2 %.3e" % (name, err), flush=True)
+        assert err <= 1e-12, (name, err)
+    sample = np.sort(rng.choice(n, size=1 << 20, replace=False))   # sorted positions
+    vref = np.zeros((sample.size, 3))
+    gref = np.zeros((sample.size, 9))
+    ps = pts_np[perm_np[sample]]
+    oracle.g2p(mesh, 2, 1, 1.0, u_np.copy(), ps, vref, threads=threads)
+    oracle.g2p(mesh, 2, 3, 1.0, u_np.copy(), ps, gref, threads=threads)
+    idx = torch.tensor(sample, device="cuda")
+    ev = rel_l2(vel[idx].cpu().numpy(), vref)
+    eg = rel_l2(grad[idx].reshape(-1, 9).cpu().numpy(), gref)
+    print("  G2P velocity %.3e, velocity gradient %.3e (1 M sampled particles)" % (ev, eg))
+    assert ev <= 1e-12 and eg <= 1e-12
+
+
+def _clustered(n, cells, seed, sigma, num_cluster=32):
+    from cabana_amd import mpm
+    return mpm.clustered_positions(n, cells, num_cluster=num_cluster, sigma=sigma, seed=seed)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_config5_reduced_clustered_against_oracle(ca, oracle, dtype):
+    """32^3 cells, 400 k particles in 8 Gaussian clusters (sigma 1-3 cells)
+    over a 10 % background, so the densest cells hold hundreds of particles:
+    one step's mass and momentum P2G and velocity G2P (the transfers of the
+    MPM loop, binned and fused) against the oracle."""
+    cells, n = 32, 400_000
+    x = _clustered(n, cells, seed=5151, sigma=(1.0, 3.0), num_cluster=8)
+    counts = torch.bincount(((x[:, 0].long() * cells + x[:, 1].long()) * cells +
+                             x[:, 2].long()), minlength=cells ** 3)
+    print("reduced clustered case: max %d particles per cell" % int(counts.max()), flush=True)
+    assert int(counts.max()) >= 200
+    lg, mesh = _mesh(ca, oracle, cells)
+    lcl = ca.createLinkedCellList(x, [1.0] * 3, [0.0] * 3, [float(cells)] * 3)
+    xs = x[lcl.permutes].contiguous()
+    rng = np.random.default_rng(77)
+    m_np = rng.uniform(0.5, 1.5, size=n)
+    q_np = m_np[:, None] * rng.normal(size=(n, 3))
+    m = torch.tensor(m_np, dtype=dtype, device="cuda")
+    q = torch.tensor(q_np, dtype=dtype, device="cuda")
+    arr = lambda d: ca.createArray("g", ca.createArrayLayout(lg, d, ca.Node()), dtype=dtype)  # noqa
+    gmass, gmom, u = arr(1), arr(3), arr(3)
+    halo = ca.createHalo(ca.NodeHaloPattern(), 1, gmass)
+    ca.p2g([ca.createScalarValueP2G(m, 1.0), ca.createVectorValueP2G(q, 1.0)], xs, n,
+           ca.Spline(2), halo, [gmass, gmom], bins=lcl)
+    u_np = rng.normal(size=mesh.num_node + (3,))
+    u.view().copy_(torch.tensor(u_np, dtype=dtype))
+    vel = torch.zeros(n, 3, dtype=dtype, device="cuda")
+    grad = torch.zeros(n, 3, 3, dtype=dtype, device="cuda")
+    ca.g2p([u, u], halo, xs, n, ca.Spline(2),
+           [ca.createVectorValueG2P(vel, 1.0), ca.createVectorGradientG2P(grad, 1.0)], bins=lcl)
+    assert ca.check_domain() == 0
+    xs_np = xs.cpu().numpy()
+    own = mesh.owned_slices()
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    for name, got, op, vals, nd in (("mass", gmass, 0, m_np[:, None], 1),
+                                    ("momentum", gmom, 1, q_np, 3)):
+        ref = mesh.grid(nd)
+        oracle.p2g(mesh, 2, op, 1.0, xs_np, vals, ref)
+        err = rel_l2(got.view().double().cpu().numpy()[own], ref[own])
+        print("  P2G %s rel-L2 %.3e" % (name, err))
+        assert err <= tol, (name, err)
+    vref = np.zeros((n, 3))
+    gref = np.zeros((n, 9))
+    oracle.g2p(mesh, 2, 1, 1.0, u_np.copy(), xs_np, vref)
+    oracle.g2p(mesh, 2, 3, 1.0, u_np.copy(), xs_np, gref)
+    ev = rel_l2(vel.double().cpu().numpy(), vref)
+    eg = rel_l2(grad.double().reshape(-1, 9).cpu().numpy(), gref)
+    print("  G2P velocity %.3e, gradient %.3e" % (ev, eg))
+    assert ev <= tol and eg <= tol
+
+
+def _config5_state(n, cells, seed=505):
+    x0 = _clustered(n, cells, seed=seed, sigma=(4.0, 16.0))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed + 1)
+    v0 = 0.5 * torch.randn(n, 3, dtype=torch.float64, device="cuda", generator=g)
+    m0 = torch.rand(n, dtype=torch.float64, device="cuda", generator=g) + 0.5
+    return x0, v0, m0
+
+
+def test_config5_full_shape(ca):
+    """256^3 cells, 2^26 clustered particles, FP32-mixed, quadratic:
+    (1) mass conservation at each of 10 loop steps (sum of the owned P2G mass
+        nodes, in FP64, against the sum of the particle masses) to 1e-6;
+    (2) one step's fields (P2G mass and momentum, G2P velocity and velocity
+        gradient) and gradients (positions, velocities) against the FP64
+        path on the same inputs, per field, to 1e-5 relative L2;
+    (3) checkpointed (every 2 steps) against plain gradients over 4 steps."""
+    from cabana_amd import mpm
+    cells, n = 256, 1 << 26
+    box = mpm.PeriodicBox(cells)
+    x0, v0, m0 = _config5_state(n, cells)
+    m32 = m0.float()
+    msum = m32.double().sum().item()
+    print("config 5: %d particles, 10 steps" % n, flush=True)
+    # (1) mass conservation, every step.
+    with torch.no_grad():
+        x, v, m = x0.clone(), v0.float(), m32
+        for step in range(10):
+            rec = []
+            x, v, m, _ = mpm.explicit_step(box, x, v, m, record=rec)
+            tot = rec[0][0].double().sum().item()
+            err = abs(tot / msum - 1.0)
+            print("  step %d: owned mass / particle mass - 1 = %.2e" % (step, err), flush=True)
+            assert err <= 1e-6
+            del rec
+    # (2) one step, FP32-mixed against FP64.
+    res = {}
+    for dt in (torch.float64, torch.float32):
+        xr = x0.clone().requires_grad_()
+        vr = v0.to(dt).requires_grad_()
+        rec = []
+        x1, v1, _, gv = mpm.explicit_step(box, xr, vr, m0.to(dt), record=rec)
+        loss = (v1.double() ** 2).sum() + 1e-3 * x1.sum() + 1e-2 * gv.double().sum()
+        loss.backward()
+        res[dt] = {"p2g_mass": rec[0][0].double(), "p2g_momentum": rec[0][1].double(),
+                   "g2p_velocity": v1.detach().double(), "g2p_velocity_grad": gv.detach().double(),
+                   "grad_positions": xr.grad.double(), "grad_velocities": vr.grad.double()}
+        del xr, vr, x1, v1, gv, loss, rec
+        torch.cuda.empty_cache()
+    for k in res[torch.float64]:
+        a, b = res[torch.float32][k], res[torch.float64][k]
+        e = (torch.linalg.norm(a - b) / torch.linalg.norm(b)).item()
+        print("  FP32-mixed vs FP64 %s: rel-L2 %.3e, max abs %.3e (max %.3e)"
+              % (k, e, (a - b).abs().max().item(), b.abs().max().item()), flush=True)
+        assert e <= 1e-5, (k, e)
+    del res
+    torch.cuda.empty_cache()
+    # (3) checkpointed against plain, 4 steps.
+    grads = []
+    for every in (0, 2):
+        xr = x0.clone().requires_grad_()
+        vr = v0.float().requires_grad_()
+        xs, vs, _, _ = mpm.run_loop(box, xr, vr, m32, num_steps=4, checkpoint_every=every)
+        ((vs.double() ** 2).sum() + 1e-3 * xs.sum()).backward()
+        grads.append((xr.grad.double(), vr.grad.double()))
+        del xr, vr, xs, vs
+        torch.cuda.empty_cache()
+    for name, a, b in (("positions", grads[1][0], grads[0][0]),
+                       ("velocities", grads[1][1], grads[0][1])):
+        e = (torch.linalg.norm(a - b) / torch.linalg.norm(b)).item()
+        print("  checkpointed vs plain gradient (%s): rel-L2 %.3e" % (name, e), flush=True)
+        assert e <= 1e-5
+    assert ca.check_domain() == 0
