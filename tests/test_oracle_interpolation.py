@@ -316,6 +316,94 @@ def test_halo_gather_scatter_pattern(oracle, width):
     check_scatter_pattern(mesh, grid, width)
 
 
+def check_gather_pattern(mesh, grid, width, periodic):
+    """tstHalo3d.hpp:52-117 (checkGather) on one rank: after the gather a node
+    holds 1 if it is owned or lies within `width` (+1 on the high side, the
+    Node halo pad) of the owned nodes in every dimension, 0 otherwise; the
+    boundary halo of a non-periodic dimension stays 0."""
+    own = mesh.owned_slices()
+    for i in range(mesh.num_node[0]):
+        for j in range(mesh.num_node[1]):
+            for k in range(mesh.num_node[2]):
+                idx = (i, j, k)
+                boundary = any(not periodic[d] and not (own[d].start <= idx[d] < own[d].stop)
+                               for d in range(3))
+                outside = any(idx[d] < own[d].start - width or
+                              idx[d] >= own[d].stop + width + 1 for d in range(3))
+                want = 0.0 if (boundary or outside) else 1.0
+                assert np.all(grid[i, j, k] == want), (idx, grid[i, j, k], want)
+
+
+def check_scatter_pattern_periodicity(mesh, grid, width, periodic):
+    """tstHalo3d.hpp:123-177 (checkScatter) on one rank: an owned node holds
+    2^n, n = the periodic dimensions in which it lies in a halo-shared band
+    (a non-periodic dimension of a single rank has no neighbour)."""
+    own = mesh.owned_slices()
+    for i in range(own[0].start, own[0].stop):
+        for j in range(own[1].start, own[1].stop):
+            for k in range(own[2].start, own[2].stop):
+                num = 0
+                for d, x in enumerate((i, j, k)):
+                    if not periodic[d]:
+                        continue
+                    if x < own[d].start + width + 1 or x >= own[d].stop - width:
+                        num += 1
+                assert np.all(grid[i, j, k] == 2.0 ** num), ((i, j, k), grid[i, j, k], num)
+
+
+@pytest.mark.parametrize("periodic", [(True, True, True), (False, True, False),
+                                      (True, False, True), (False, False, False)])
+@pytest.mark.parametrize("width", [1, 2, 3])
+def test_halo_gather_scatter_pattern_periodicity(oracle, width, periodic):
+    """tstHalo3d.hpp:181-230 geometry with periodic and non-periodic
+    dimensions: the non-periodic branch of the oracle's halo exchange
+    (no neighbour across a non-periodic boundary) against the reference's
+    checkGather / checkScatter expectations."""
+    cell = 0.23
+    low = np.array([1.2, 3.3, -2.8])
+    high = low + cell * np.array([17, 20, 21])
+    mesh = oracle.OMesh(low, high, cell, halo=3, periodic=periodic)
+    grid = mesh.grid(4)
+    grid[mesh.owned_slices()] = 1.0
+    oracle.halo_gather(mesh, grid, width)
+    check_gather_pattern(mesh, grid, width, periodic)
+    oracle.halo_scatter_sum(mesh, grid, width)
+    check_scatter_pattern_periodicity(mesh, grid, width, periodic)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_openmp_g2p_and_adjoints_equal_serial(oracle, order):
+    """The threaded G2P and adjoints (the CPU baseline's code) against the
+    Serial ones, every operator."""
+    mesh, pts, rng = random_case(oracle, order, n=3000, seed=78, num_cell=(8, 8, 8))
+    n = len(pts)
+    for op in range(5):
+        nc, nd = G2P_SHAPE[op]
+        g = rng.normal(size=mesh.num_node + (nd,))
+        a = np.zeros((n, nc))
+        b = np.zeros((n, nc))
+        oracle.g2p(mesh, order, op, 0.7, g.copy(), pts, a)
+        oracle.g2p(mesh, order, op, 0.7, g.copy(), pts, b, threads=4)
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12)
+        vb = rng.normal(size=(n, nc))
+        ga, gb = mesh.grid(nd), mesh.grid(nd)
+        xa, xb = np.zeros((n, 3)), np.zeros((n, 3))
+        oracle.g2p_adjoint(mesh, order, op, 0.7, g.copy(), pts, vb, ga, xa)
+        oracle.g2p_adjoint(mesh, order, op, 0.7, g.copy(), pts, vb, gb, xb, threads=4)
+        np.testing.assert_allclose(ga, gb, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(xa, xb, rtol=1e-12, atol=1e-12)
+    for op in range(5):
+        nc, nd = P2G_SHAPE[op]
+        vals = rng.normal(size=(n, nc))
+        gbar = rng.normal(size=mesh.num_node + (nd,))
+        va, vb2 = np.zeros((n, nc)), np.zeros((n, nc))
+        xa, xb = np.zeros((n, 3)), np.zeros((n, 3))
+        oracle.p2g_adjoint(mesh, order, op, -0.4, pts, vals, gbar.copy(), va, xa)
+        oracle.p2g_adjoint(mesh, order, op, -0.4, pts, vals, gbar.copy(), vb2, xb, threads=4)
+        np.testing.assert_allclose(va, vb2, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(xa, xb, rtol=1e-12, atol=1e-12)
+
+
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_openmp_equals_serial(oracle, order):
     mesh, pts, rng = random_case(oracle, order, n=2000, seed=77, num_cell=(8, 8, 8))
