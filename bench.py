@@ -28,6 +28,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 NODES = 131 ** 3
+# Whole-step algorithmic bytes per particle of the fwd+adjoint MPM transfer
+# set at 128^3 x 16 ppc (positions FP64 once per pass, particle fields read
+# once, += outputs read and written, grids amortised at 0.067 nodes per
+# particle): 450 B in the FP32-mixed form, 804 B in FP64.
+STEP_BYTES_PER_PARTICLE = {"fp32": 450.0, "fp64": 804.0}
 
 
 def parse():
@@ -40,7 +45,11 @@ def parse():
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-sizes", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-threads", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--runs", type=int, default=10,
+                    help="individually fenced steps for the min / median figures")
     ap.add_argument("--mode", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the differentiation and checkpointed-loop figures")
@@ -296,6 +305,20 @@ def time_steps(torch, fn, steps, warmup):
     return t0.elapsed_time(t1) / steps
 
 
+def step_distribution(torch, fn, runs):
+    """Min and median of `runs` individually fenced steps (HIP events on the
+    current stream around each step)."""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(runs)]
+    for a, b in ev:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    ms = sorted(a.elapsed_time(b) for a, b in ev)
+    return {"runs": runs, "min_ms": ms[0], "median_ms": ms[len(ms) // 2]}
+
+
 def kernel_profile(native, torch, fn, steps):
     """Average HIP-event duration of every kernel family over `steps` steps
     (events are recorded on the launch stream around each launch)."""
@@ -341,20 +364,44 @@ def measured_traffic(kernel, workload_tag):
     return None
 
 
-def cpu_baseline(cells, ppc, order, budget_s):
-    """The oracle (OpenMP restatement of Cabana's p2g/g2p with ScatterView
-    duplication, plus OpenMP adjoints) on the host cores, on a bounded sample
-    of particles of the same workload; returns particles/s for fwd+adj."""
+def host_facts():
+    """CPU model, logical CPUs, physical cores, and the CPUs this process may
+    run on (the box's share)."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "physical_cores": len(cores) or None,
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_child(cells, ppc, order, threads, sizes, runs):
+    """Runs in a child process (no GPU): the oracle's fwd+adj operator set on
+    samples of the workload; per size, `runs` timed runs after one warm-up."""
     import numpy as np
 
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    threads = max(1, min(threads, 16))
     L = float(cells)
     mesh = oracle.OMesh([0.0] * 3, [L] * 3, 1.0, halo=1)
     rng = np.random.default_rng(20260102)
 
-    def run(n):
+    def run(n, thr):
         pts = rng.uniform(0, L, size=(n, 3))
         m = rng.uniform(0.5, 1.5, size=(n, 1))
         q = rng.normal(size=(n, 3))
@@ -370,29 +417,68 @@ def cpu_baseline(cells, ppc, order, budget_s):
         mb, qb, sb = np.zeros((n, 1)), np.zeros((n, 3)), np.zeros((n, 9))
         gbm, gbq, gbs = mesh.grid(1), mesh.grid(3), mesh.grid(3)
         t = time.perf_counter()
-        oracle.p2g(mesh, order, 0, 1.0, pts, m, gm, threads=threads)
-        oracle.p2g(mesh, order, 1, 1.0, pts, q, gq, threads=threads)
-        oracle.p2g(mesh, order, 4, -1.0, pts, sig, gs, threads=threads)
-        oracle.g2p(mesh, order, 1, 1.0, u, pts, vel, threads=threads)
-        oracle.g2p(mesh, order, 3, 1.0, u, pts, lg, threads=threads)
-        oracle.g2p_adjoint(mesh, order, 1, 1.0, u, pts, ub_v, gb_u, xbar, threads=threads)
-        oracle.g2p_adjoint(mesh, order, 3, 1.0, u, pts, ub_l, gb_u, xbar, threads=threads)
-        oracle.p2g_adjoint(mesh, order, 0, 1.0, pts, m, gbm, mb, xbar, threads=threads)
-        oracle.p2g_adjoint(mesh, order, 1, 1.0, pts, q, gbq, qb, xbar, threads=threads)
-        oracle.p2g_adjoint(mesh, order, 4, -1.0, pts, sig, gbs, sb, xbar, threads=threads)
+        oracle.p2g(mesh, order, 0, 1.0, pts, m, gm, threads=thr)
+        oracle.p2g(mesh, order, 1, 1.0, pts, q, gq, threads=thr)
+        oracle.p2g(mesh, order, 4, -1.0, pts, sig, gs, threads=thr)
+        oracle.g2p(mesh, order, 1, 1.0, u, pts, vel, threads=thr)
+        oracle.g2p(mesh, order, 3, 1.0, u, pts, lg, threads=thr)
+        oracle.g2p_adjoint(mesh, order, 1, 1.0, u, pts, ub_v, gb_u, xbar, threads=thr)
+        oracle.g2p_adjoint(mesh, order, 3, 1.0, u, pts, ub_l, gb_u, xbar, threads=thr)
+        oracle.p2g_adjoint(mesh, order, 0, 1.0, pts, m, gbm, mb, xbar, threads=thr)
+        oracle.p2g_adjoint(mesh, order, 1, 1.0, pts, q, gbq, qb, xbar, threads=thr)
+        oracle.p2g_adjoint(mesh, order, 4, -1.0, pts, sig, gbs, sb, xbar, threads=thr)
         return time.perf_counter() - t
 
-    # The per-thread grid copies make small samples cost more per particle:
-    # grow the sample until it takes at least half the budget.
-    n = 1 << 18
-    t = run(n)
-    while t < 0.5 * budget_s and n < cells ** 3 * ppc:
-        n = int(min(cells ** 3 * ppc, n * max(2.0, 0.8 * budget_s / max(t, 1e-3))))
-        t = run(n)
-    return {"value": n / t, "unit": "particles/s", "cores": threads, "kind": "port",
-            "sample": "%d of %d particles, full 128^3 grid, FP64, same fwd+adj operator set "
-                      "(oracle: C restatement, OpenMP with per-thread grid copies), %.1f s"
-                      % (n, cells ** 3 * ppc, t)}
+    out = []
+    for n, thr in zip(sizes, threads):
+        run(min(n, 1 << 16), thr)
+        ts = sorted(run(n, thr) for _ in range(runs))
+        out.append({"particles": n, "threads": thr or 1, "omp": bool(thr), "runs": runs,
+                    "min_s": ts[0], "median_s": ts[len(ts) // 2],
+                    "particles_per_s_median": n / ts[len(ts) // 2]})
+    print(json.dumps(out))
+
+
+def cpu_baseline(cells, ppc, order):
+    """The oracle (OpenMP restatement of Cabana's p2g/g2p with ScatterView
+    duplication: per-thread grid copies, then a sum; plus OpenMP adjoints)
+    on the host cores, in a child process started before this process
+    touches the GPU, with OMP_PROC_BIND=spread.  The thread count is the
+    OMP_NUM_THREADS this process was given (the box's CPU share), at most
+    the physical cores; a 1-thread Serial run (the oracle's Serial loop
+    order) is timed beside it.  Bounded samples of the workload's particles
+    over its full grid; two OpenMP sample sizes show the per-particle cost
+    does not depend on the sample."""
+    import subprocess
+    facts = host_facts()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0")) or facts["affinity_cpus"]
+    threads = max(1, min(share, facts["physical_cores"] or share, facts["affinity_cpus"]))
+    env = dict(os.environ)
+    env["OMP_PROC_BIND"] = "spread"
+    env["OMP_PLACES"] = "cores"
+    env["OMP_NUM_THREADS"] = str(threads)
+    sizes = [1 << 22, 1 << 23, 1 << 17]
+    thr = [threads, threads, 0]
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", "--cells", str(cells),
+           "--ppc", str(ppc), "--order", str(order), "--cpu-sizes",
+           ",".join(map(str, sizes)), "--cpu-threads", ",".join(map(str, thr))]
+    res = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True, check=True)
+    rows = json.loads(res.stdout.strip().splitlines()[-1])
+    omp = [r for r in rows if r["omp"]]
+    ser = [r for r in rows if not r["omp"]]
+    best = omp[-1]
+    return {"value": best["particles_per_s_median"], "unit": "particles/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d of %d particles (uniform, full %d^3 grid), FP64, the same fwd+adj "
+                      "operator set; oracle = C restatement with OpenMP per-thread grid "
+                      "copies (ScatterView duplication); median of %d runs"
+                      % (best["particles"], cells ** 3 * ppc, cells, best["runs"]),
+            "omp_env": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "spread",
+                        "OMP_PLACES": "cores"},
+            "host": facts,
+            "runs": rows,
+            "serial_1_thread": ser[0]["particles_per_s_median"] if ser else None,
+            "label": "Cabana-equivalent restatement (Cabana itself not buildable: no Kokkos)"}
 
 
 def copy_bandwidth(torch, mib=2048, reps=5):
@@ -417,10 +503,23 @@ def copy_bandwidth(torch, mib=2048, reps=5):
 
 def main():
     args = parse()
+    if args.cpu_child:
+        cpu_child(args.cells, args.ppc, args.order,
+                  [int(t) for t in args.cpu_threads.split(",")],
+                  [int(n) for n in args.cpu_sizes.split(",")], runs=3)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only (this build has no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    # The CPU baseline runs first, in a child process, before this process
+    # initialises the GPU.
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cells, args.ppc, args.order)
+        print("[bench] cpu: %.3e particles/s on %d threads (Serial %.3e)"
+              % (cpu["value"], cpu["cores"], cpu["serial_1_thread"] or 0.0),
+              file=sys.stderr, flush=True)
     import torch
 
     import cabana_amd as ca
@@ -440,13 +539,14 @@ def main():
     for label, dtype in modes:
         w = Workload(torch, ca, dtype, args.cells, args.ppc, args.order)
         ms = time_steps(torch, w.step, args.steps, args.warmup)
+        dist = step_distribution(torch, w.step, args.runs)
         psteps = max(2, min(args.steps, 5))
         prof = kernel_profile(_native, torch, w.step, psteps)
         outputs[label] = w.outputs_of_one_step()
         ms_sort = time_steps(torch, w.sort_step, 2, 1)
         ca.check_domain()
         res[label] = {"ms": ms, "rate": w.n / (ms * 1e-3), "kernels": prof,
-                      "prof_steps": psteps, "n": w.n, "sort_ms": ms_sort}
+                      "prof_steps": psteps, "n": w.n, "sort_ms": ms_sort, "dist": dist}
         print("[bench] %s: %.3f ms/step, %.3e particles/s, sort %.3f ms" %
               (label, ms, w.n / (ms * 1e-3), ms_sort), file=sys.stderr, flush=True)
         del w
@@ -468,6 +568,12 @@ def main():
                      for k, v in kernels.items() if algorithmic_bytes(k, 1, s, 1)) / head["prof_steps"]
     roofline["step_algorithmic_bytes"] = step_bytes
     roofline["step_frac"] = step_bytes / (head["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+    # The whole step on the fixed per-particle byte model (independent of
+    # how the step is split into kernels), for each precision measured.
+    for label, r in res.items():
+        b = STEP_BYTES_PER_PARTICLE[label] * r["n"]
+        roofline["step_frac_model_%s" % label] = b / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        roofline["step_bytes_model_%s" % label] = b
     extra = {}
     if "fp32" in outputs and "fp64" in outputs:
         extra["precision"] = {
@@ -486,9 +592,6 @@ def main():
         torch.cuda.empty_cache()
         for k, v in extra.items():
             print("[bench] %s: %s" % (k, json.dumps(v)), file=sys.stderr, flush=True)
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(args.cells, args.ppc, args.order, args.cpu_seconds)
     out = {
         "metric": "P2G+G2P particles/s at 128^3x16ppc quadratic: FP32-mixed vs FP64, "
                   "fwd+adjoint",
@@ -509,9 +612,14 @@ def main():
                    "cells": args.cells, "ppc": args.ppc, "order": args.order},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "step_ms_min": head["dist"]["min_ms"],
+        "step_ms_median": head["dist"]["median_ms"],
+        "step_runs": head["dist"]["runs"],
     }
     if "fp64" in res:
-        out["fp64"] = {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms"]}
+        out["fp64"] = {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms"],
+                       "step_ms_min": res["fp64"]["dist"]["min_ms"],
+                       "step_ms_median": res["fp64"]["dist"]["median_ms"]}
         if "fp32" in res:
             out["fp32_over_fp64"] = res["fp32"]["rate"] / res["fp64"]["rate"]
     copy_gbs = copy_bandwidth(torch)

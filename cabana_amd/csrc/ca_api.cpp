@@ -535,8 +535,6 @@ int64_t max_tile_groups()
         x = 4096;
     return (int64_t)( x / 8 * 8 );
 }
-// Longest run of one base cell that one scatter lane accumulates.
-constexpr int SCATTER_SUB_LEN = 9;
 
 struct TileShape
 {
@@ -575,7 +573,11 @@ size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng, int nslot,
     const size_t rs = (size_t)record_stride( rec_len, (int)real );
     b = a16( 8 * (size_t)dt * wn );
     b = a16( b + 4 * ( nbase + 1 ) );
-    b = a16( b + 4 * (size_t)sh.max_tasks );
+    b = a16( b + 4 * (size_t)sh.max_tasks ); // sorted task table
+    b = a16( b + 4 * (size_t)sh.max_tasks ); // unsorted tasks
+    b = a16( b + 4 * (size_t)sh.max_tasks ); // their length bucket and rank
+    b = a16( b + 4 * 2 * SCATTER_TASK_MAX ); // length histogram and offsets
+    b = a16( b + 2 * (size_t)sh.cap );
     b = a16( b + real * rs * (size_t)sh.cap );
     for ( int buf = 0; buf < 2; ++buf )
     {
@@ -590,8 +592,9 @@ size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng, int nslot,
 // Brick shape and part capacity.  Gathers use the largest cubic brick whose
 // windows fit three workgroups per CU, with parts of ~1.25x the mean brick
 // population.  Scatters sort a whole part in LDS: the brick is the largest
-// near-cubic one whose expected population (x1.3) fits a part that, with the
-// FP64 window, keeps two workgroups per CU; denser bricks are cut into
+// near-cubic one whose expected population (x1.2) fits a part that, with the
+// FP64 window, keeps three workgroups per CU for FP32 records and two for
+// FP64 ones (the kernel's register budget); denser bricks are cut into
 // several parts.
 bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int rec_len,
                    int order, double mean_ppc, TileShape& out )
@@ -615,10 +618,10 @@ bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int 
         }
         return false;
     }
-    const size_t budget = 80 * 1024;
+    const size_t budget = fp64 ? 78 * 1024 : 52 * 1024;
     auto with_cap = [&]( TileShape sh, int cap ) {
         sh.cap = cap;
-        sh.max_tasks = base_cells( sh.T, order ) + cap / SCATTER_SUB_LEN + 1;
+        sh.max_tasks = base_cells( sh.T, order ) + cap / SCATTER_TASK_MAX + 1;
         return sh;
     };
     bool found = false;
@@ -631,10 +634,10 @@ bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int 
                 const int vol = a * b * c;
                 if ( vol <= best_vol )
                     continue;
-                int cap = (int)( 1.3 * mean_ppc * vol ) + 32;
+                int cap = (int)( 1.2 * mean_ppc * vol ) + 32;
                 cap = ( cap + 63 ) / 64 * 64;
                 cap = cap < 256 ? 256 : cap;
-                if ( cap > SCATTER_CAP_MAX )
+                if ( cap > scatter_cap_max( fp64 ) )
                     continue;
                 TileShape sh = with_cap( { { a, b, c }, 0, 0 }, cap );
                 if ( tile_lds_bytes( kind, fp64, slot_ng, nslot, rec_len, order, sh ) <=
@@ -648,7 +651,7 @@ bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int 
     if ( !found )
     {
         // Dense: the smallest brick with the largest part that fits.
-        for ( int cap = SCATTER_CAP_MAX; cap >= 64; cap -= 64 )
+        for ( int cap = scatter_cap_max( fp64 ); cap >= 64; cap -= 64 )
         {
             TileShape sh = with_cap( { { 2, 2, 2 }, 0, 0 }, cap );
             if ( tile_lds_bytes( kind, fp64, slot_ng, nslot, rec_len, order, sh ) <= budget )
@@ -699,7 +702,6 @@ bool make_tiles( TileKind kind, const MeshDev& m, int order, bool fp64,
     t.max_win_nodes = t.win[0] * t.win[1] * t.win[2];
     t.max_base_cells = base_cells( sh.T, order );
     t.cap = sh.cap;
-    t.sub_len = SCATTER_SUB_LEN;
     t.max_tasks = sh.max_tasks;
     t.bin_begin = bins->begin;
     // Window walk order: largest grid stride first.
@@ -724,7 +726,8 @@ bool make_tiles( TileKind kind, const MeshDev& m, int order, bool fp64,
 // reuses it without launching anything.
 cabana_amd_status make_parts( cabana_amd_context* ctx, const MeshDev& m,
                               const cabana_amd_binning* bins, TileArgs& t,
-                              hipStream_t s )
+                              hipStream_t s, void* fixed = nullptr,
+                              size_t fixed_bytes = 0 )
 {
     using PartTable = cabana_amd_context::PartTable;
     const int64_t nbricks = (int64_t)t.ntile[0] * t.ntile[1] * t.ntile[2];
@@ -734,7 +737,7 @@ cabana_amd_status make_parts( cabana_amd_context* ctx, const MeshDev& m,
     t.grid = (int32_t)( grid < max_tile_groups() ? grid : max_tile_groups() );
     PartTable* pt = nullptr;
     uint64_t gen_c = 0, gen_o = 0;
-    if ( bins->serial != 0 )
+    if ( bins->serial != 0 && !fixed )
     {
         if ( bins->serial > g_serials.load() )
             return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
@@ -772,7 +775,17 @@ cabana_amd_status make_parts( cabana_amd_context* ctx, const MeshDev& m,
     const size_t o_pt = carve( sizeof( int2 ) * max_parts );
     const size_t o_n = carve( sizeof( int32_t ) );
     char* base = nullptr;
-    if ( pt )
+    if ( fixed )
+    {
+        // The caller's fixed scratch (no allocation, no table cache).
+        if ( off > fixed_bytes )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "the brick part table needs " + std::to_string( off ) +
+                             " bytes; the no-allocation scratch holds " +
+                             std::to_string( fixed_bytes ) );
+        base = static_cast<char*>( fixed );
+    }
+    else if ( pt )
     {
         pt->serial = 0; // empty until rebuilt below
         if ( pt->bytes < off )
@@ -850,7 +863,8 @@ bool pf_contig( const PFieldDev& f ) { return !f.data || f.vl == 1; }
 cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
                                 int plan, ScatterArgs a, int64_t n,
                                 const cabana_amd_binning* bins,
-                                hipStream_t s )
+                                hipStream_t s, void* fixed = nullptr,
+                                size_t fixed_bytes = 0 )
 {
     int forms[MAX_TERMS], ng[MAX_TERMS];
     for ( int i = 0; i < a.nterms; ++i )
@@ -885,7 +899,7 @@ cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
     }
     if ( bins->end > bins->begin )
     {
-        CA_CHECK( make_parts( ctx, a.mesh, bins, t, s ) );
+        CA_CHECK( make_parts( ctx, a.mesh, bins, t, s, fixed, fixed_bytes ) );
         CA_HIP( ctx, run_scatter_cells( ctx, order, fp64, plan, a, t, lds, s ),
                 "scatter (cells)" );
     }
@@ -1074,11 +1088,17 @@ extern "C" cabana_amd_status cabana_amd_check_domain( cabana_amd_context* ctx,
 // P2G / G2P
 // ---------------------------------------------------------------------------
 
-extern "C" cabana_amd_status cabana_amd_p2g(
-    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
-    const cabana_amd_particle_field* points, int32_t num_terms,
-    const cabana_amd_p2g_term* terms, int32_t halo_width,
-    const cabana_amd_binning* bins, cabana_amd_stream stream )
+namespace
+{
+// p2g: every term added into its grid, then each distinct grid folded once.
+// With `fixed`, a tiled scatter builds its part table in that memory (of
+// fixed_bytes) instead of the context's allocations; nothing else the call
+// uses is allocated here either (terms live on the stack).
+cabana_amd_status p2g_impl( cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+                            int32_t order, const cabana_amd_particle_field* points,
+                            int32_t num_terms, const cabana_amd_p2g_term* terms,
+                            int32_t halo_width, const cabana_amd_binning* bins,
+                            cabana_amd_stream stream, void* fixed, size_t fixed_bytes )
 {
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
@@ -1098,7 +1118,7 @@ extern "C" cabana_amd_status cabana_amd_p2g(
 
     ScatterArgs a;
     std::memset( &a, 0, sizeof( a ) );
-    std::vector<ScatterTermDev> T( num_terms );
+    ScatterTermDev T[MAX_CALL_TERMS];
     a.mesh = m;
     a.skipped = ctx->skipped;
     CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, n, false,
@@ -1137,7 +1157,7 @@ extern "C" cabana_amd_status cabana_amd_p2g(
             CA_CHECK( scatter_pass( ctx, order, fp64,
                                     mpm ? PLAN_MPM_P2G
                                         : ( mm ? PLAN_MASS_MOMENTUM : PLAN_SINGLE ),
-                                    a, n, bins, s ) );
+                                    a, n, bins, s, fixed, fixed_bytes ) );
         }
         else
         {
@@ -1147,7 +1167,7 @@ extern "C" cabana_amd_status cabana_amd_p2g(
                 b.nterms = 1;
                 b.t[0] = T[i];
                 CA_CHECK( scatter_pass( ctx, order, fp64, PLAN_SINGLE, b, n,
-                                        bins, s ) );
+                                        bins, s, fixed, fixed_bytes ) );
             }
         }
     }
@@ -1164,6 +1184,50 @@ extern "C" cabana_amd_status cabana_amd_p2g(
     }
     return CABANA_AMD_OK;
 }
+} // namespace
+
+extern "C" cabana_amd_status cabana_amd_p2g(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
+    const cabana_amd_particle_field* points, int32_t num_terms,
+    const cabana_amd_p2g_term* terms, int32_t halo_width,
+    const cabana_amd_binning* bins, cabana_amd_stream stream )
+{
+    return p2g_impl( ctx, mesh, order, points, num_terms, terms, halo_width, bins, stream,
+                     nullptr, 0 );
+}
+
+namespace cabana_amd
+{
+cabana_amd_status p2g_fixed_scratch( cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+                                     int32_t order, const cabana_amd_particle_field* points,
+                                     int32_t num_terms, const cabana_amd_p2g_term* terms,
+                                     int32_t halo_width, const cabana_amd_binning* bins,
+                                     cabana_amd_stream stream, void* scratch,
+                                     size_t scratch_bytes )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    int cur = -1;
+    if ( hipGetDevice( &cur ) != hipSuccess || cur != ctx->device )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "the context's device must be the current device" );
+    if ( !scratch )
+        return fail( ctx, CABANA_AMD_ERROR_DEVICE,
+                     "the fixed scratch of this device has no address" );
+    // The kernel timer creates events: it stays off for this call.
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;
+    const cabana_amd_status st = p2g_impl( ctx, mesh, order, points, num_terms, terms,
+                                           halo_width, bins, stream, scratch, scratch_bytes );
+    ctx->profiling = prof;
+    return st;
+}
+
+cabana_amd_status fail_with( cabana_amd_context* ctx, cabana_amd_status st, const char* msg )
+{
+    return fail( ctx, st, msg );
+}
+} // namespace cabana_amd
 
 extern "C" cabana_amd_status cabana_amd_g2p(
     cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,

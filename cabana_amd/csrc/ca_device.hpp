@@ -9,6 +9,19 @@
 namespace cabana_amd
 {
 
+// Calls f(std::integral_constant<int, I>) for I = 0 .. N-1.
+template <class F, std::size_t... I>
+__device__ __forceinline__ void static_for_impl( F&& f,
+                                                 std::index_sequence<I...> )
+{
+    ( f( std::integral_constant<int, (int)I>{} ), ... );
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for( F&& f )
+{
+    static_for_impl( f, std::make_index_sequence<N>{} );
+}
+
 // Logical coordinate of a position, Spline<N>::mapToLogicalGrid
 // (Cabana_Grid_Splines.hpp :141-145 primal for odd orders, :239-244 dual for
 // even ones).  Rounded exactly as written: the cell index derived from it
@@ -58,78 +71,77 @@ __device__ __forceinline__ bool spline_frac( double x0, int num_node, int& base,
     return true;
 }
 
+// The 1-D B-spline weight of stencil node j as a polynomial of the
+// fractional offset f: w_j(f) = sum_n spline_coef(ORDER, j, n) t^n with
+// t = f - spline_shift(ORDER, j).  This is Spline<1>, <2> (dual grid) and
+// <3>::value of Cabana_Grid_Splines.hpp (:170-213, :272-324, :392-451)
+// re-expanded around a point where the weight's polynomial piece is exact
+// to evaluate: the edge nodes' weights, 0.5 (1 - f)^2 and 0.5 f^2 for the
+// quadratic spline, (1 - f)^3 / 6 and f^3 / 6 for the cubic one, become
+// a single power of t, so a weight near zero keeps its relative accuracy
+// (a sum of O(1) terms would leave an absolute error of one rounding of 1
+// in FP32, enough to change a nearly empty node's mass by O(1)).
+__host__ __device__ constexpr double spline_shift( int order, int j )
+{
+    return order == 1   ? ( j == 0 ? 1.0 : 0.0 )
+           : order == 2 ? ( j == 0 ? 1.0 : ( j == 1 ? 0.5 : 0.0 ) )
+                        : ( ( j == 0 || j == 2 ) ? 1.0 : 0.0 );
+}
+__host__ __device__ constexpr double spline_coef( int order, int j, int n )
+{
+    if ( order == 1 )
+        return n == 1 ? ( j == 0 ? -1.0 : 1.0 ) : 0.0;
+    if ( order == 2 )
+    {
+        if ( j == 1 )
+            return n == 0 ? 0.75 : ( n == 2 ? -1.0 : 0.0 );
+        return n == 2 ? 0.5 : 0.0;
+    }
+    // Cubic: t^3 / 6 at the edges (with sign), 2/3 - t^2 +- t^3 / 2 inside.
+    if ( j == 0 || j == 3 )
+        return n == 3 ? ( j == 0 ? -1.0 / 6.0 : 1.0 / 6.0 ) : 0.0;
+    return n == 0 ? 2.0 / 3.0 : ( n == 2 ? -1.0 : ( n == 3 ? ( j == 1 ? 0.5 : -0.5 ) : 0.0 ) );
+}
+
+// Value, first and second derivative (in f) of node j's weight polynomial
+// at t = f - shift, by Horner's rule with compile-time coefficients.
+template <int ORDER, int J, class Real>
+__device__ __forceinline__ void spline_node_poly( Real t, Real& w, Real& d1, Real& d2 )
+{
+    w = Real( spline_coef( ORDER, J, ORDER ) );
+#pragma unroll
+    for ( int n = ORDER - 1; n >= 0; --n )
+        w = w * t + Real( spline_coef( ORDER, J, n ) );
+    d1 = Real( ORDER * spline_coef( ORDER, J, ORDER ) );
+#pragma unroll
+    for ( int n = ORDER - 1; n >= 1; --n )
+        d1 = d1 * t + Real( n * spline_coef( ORDER, J, n ) );
+    d2 = Real( 0 );
+    if constexpr ( ORDER >= 2 )
+    {
+        d2 = Real( ORDER * ( ORDER - 1 ) * spline_coef( ORDER, J, ORDER ) );
+#pragma unroll
+        for ( int n = ORDER - 1; n >= 2; --n )
+            d2 = d2 * t + Real( n * ( n - 1 ) * spline_coef( ORDER, J, n ) );
+    }
+}
+
 // Spline<ORDER>::value/gradient (Cabana_Grid_Splines.hpp <1> :170-213, <2>
 // :272-324, <3> :392-451) and second derivatives from the fractional offset.
 template <int ORDER, class Real, bool NEED_H>
 __device__ __forceinline__ void spline_weights( Real f, Real rdx,
                                                 Spline1D<ORDER, Real>& s )
 {
-    if constexpr ( ORDER == 1 )
-    {
-        s.w[0] = Real( 1 ) - f;
-        s.w[1] = f;
-        s.g[0] = -rdx;
-        s.g[1] = rdx;
+    static_for<ORDER + 1>( [&]( auto J ) {
+        constexpr int j = J;
+        const Real t = f - Real( spline_shift( ORDER, j ) );
+        Real w, d1, d2;
+        spline_node_poly<ORDER, j, Real>( t, w, d1, d2 );
+        s.w[j] = w;
+        s.g[j] = d1 * rdx;
         if constexpr ( NEED_H )
-        {
-            s.h[0] = Real( 0 );
-            s.h[1] = Real( 0 );
-        }
-    }
-    else if constexpr ( ORDER == 2 )
-    {
-        const Real nine_eights = Real( 9.0 / 8.0 );
-        Real xn = f + Real( 0.5 );
-        s.w[0] = Real( 0.5 ) * xn * xn - Real( 1.5 ) * xn + nine_eights;
-        s.g[0] = ( xn - Real( 1.5 ) ) * rdx;
-        xn -= Real( 1 );
-        s.w[1] = -xn * xn + Real( 0.75 );
-        s.g[1] = ( Real( -2 ) * xn ) * rdx;
-        xn -= Real( 1 );
-        s.w[2] = Real( 0.5 ) * xn * xn + Real( 1.5 ) * xn + nine_eights;
-        s.g[2] = ( xn + Real( 1.5 ) ) * rdx;
-        if constexpr ( NEED_H )
-        {
-            const Real r2 = rdx * rdx;
-            s.h[0] = r2;
-            s.h[1] = Real( -2 ) * r2;
-            s.h[2] = r2;
-        }
-    }
-    else
-    {
-        const Real one_sixth = Real( 1.0 / 6.0 );
-        const Real two_thirds = one_sixth * Real( 4 );
-        const Real four_thirds = two_thirds * Real( 2 );
-        Real xn = f + Real( 1 );
-        Real xn2 = xn * xn;
-        s.w[0] = -xn * xn2 * one_sixth + xn2 - Real( 2 ) * xn + four_thirds;
-        s.g[0] = ( Real( -0.5 ) * xn * xn + Real( 2 ) * xn - Real( 2 ) ) * rdx;
-        Real h0 = -xn + Real( 2 );
-        xn -= Real( 1 );
-        xn2 = xn * xn;
-        s.w[1] = Real( 0.5 ) * xn * xn2 - xn2 + two_thirds;
-        s.g[1] = ( Real( 1.5 ) * xn * xn - Real( 2 ) * xn ) * rdx;
-        Real h1 = Real( 3 ) * xn - Real( 2 );
-        xn -= Real( 1 );
-        xn2 = xn * xn;
-        s.w[2] = Real( -0.5 ) * xn * xn2 - xn2 + two_thirds;
-        s.g[2] = ( Real( -1.5 ) * xn * xn - Real( 2 ) * xn ) * rdx;
-        Real h2 = Real( -3 ) * xn - Real( 2 );
-        xn -= Real( 1 );
-        xn2 = xn * xn;
-        s.w[3] = xn * xn2 * one_sixth + xn2 + Real( 2 ) * xn + four_thirds;
-        s.g[3] = ( Real( 0.5 ) * xn * xn + Real( 2 ) * xn + Real( 2 ) ) * rdx;
-        Real h3 = xn + Real( 2 );
-        if constexpr ( NEED_H )
-        {
-            const Real r2 = rdx * rdx;
-            s.h[0] = h0 * r2;
-            s.h[1] = h1 * r2;
-            s.h[2] = h2 * r2;
-            s.h[3] = h3 * r2;
-        }
-    }
+            s.h[j] = d2 * ( rdx * rdx );
+    } );
 }
 
 // Stencil base, weights, gradients (and second derivatives) of one
@@ -615,18 +627,6 @@ __device__ __forceinline__ void jet_dx( const Real* P, const Jet<Real>* J,
         }
         xb[e] += v;
     }
-}
-
-template <class F, std::size_t... I>
-__device__ __forceinline__ void static_for_impl( F&& f,
-                                                 std::index_sequence<I...> )
-{
-    ( f( std::integral_constant<int, (int)I>{} ), ... );
-}
-template <int N, class F>
-__device__ __forceinline__ void static_for( F&& f )
-{
-    static_for_impl( f, std::make_index_sequence<N>{} );
 }
 
 // Gather pass modes: outputs only (G2P forward), outputs and position

@@ -299,25 +299,28 @@ struct TileArgs
     int32_t max_win_nodes;       // win[0] * win[1] * win[2]
     int32_t max_base_cells;      // stencil-base cells of a brick
     int32_t cap;                 // particles per part
-    int32_t sub_len;             // scatter: particles per register task
     int32_t max_tasks;           // scatter: task-table capacity
     int32_t grid;                // workgroups launched (a multiple of 8)
     int64_t bin_begin;           // particle index of cell offset 0
 };
 
 // LDS record of one particle in the cell-sorted scatter: 3 fractional
-// offsets then the premultiplied attributes.  The record stride (in
-// elements) keeps 16-byte alignment and avoids a stride that is a multiple
-// of 32 dwords, so concurrent lanes spread over the LDS banks.
+// offsets then the premultiplied attributes, padded to 16 bytes so a record
+// is read with 128-bit LDS loads.  Records are read through a sorted index,
+// so no stride is free of bank conflicts and none is padded for it.
 __host__ __device__ constexpr int record_stride( int len, int elem_bytes )
 {
-    int dw = ( len * elem_bytes + 15 ) / 16 * 4;
-    if ( dw % 8 == 0 )
-        dw += 4;
-    return dw * 4 / elem_bytes;
+    return ( len * elem_bytes + 15 ) / 16 * 16 / elem_bytes;
 }
-// Largest part the scatter sorts in LDS (bounded by its register key array).
-constexpr int SCATTER_CAP_MAX = 768;
+// Threads of a scatter workgroup (FP32 / FP64 records) and the particles
+// each thread holds per part; their product is the largest part.
+constexpr int scatter_block( bool fp64 ) { return 256; }
+constexpr int SCATTER_SLOTS = 2;
+// Longest run of one base cell's records that one scatter task takes
+// (8 measured faster than 16 or 32 at 16 particles per cell: tasks of more
+// even length leave fewer lanes idle).
+constexpr int SCATTER_TASK_MAX = 8;
+constexpr int scatter_cap_max( bool fp64 ) { return scatter_block( fp64 ) * SCATTER_SLOTS; }
 
 // Launch-time error/timing context owned by the C-ABI layer.
 class Launcher
@@ -447,5 +450,18 @@ hipError_t launch_brick_parts( Launcher& L, const TileArgs& t,
                                int64_t* brick_start, int64_t* block_sums,
                                int64_t nblock_cap, int2* parts,
                                int32_t* num_parts, hipStream_t s );
+
+// p2g with the brick part table of a tiled scatter built in `scratch`
+// (scratch_bytes long) instead of the context's allocations; the context's
+// device must be current.  Records an error on the context and returns its
+// status.
+cabana_amd_status p2g_fixed_scratch( cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+                                     int32_t order, const cabana_amd_particle_field* points,
+                                     int32_t num_terms, const cabana_amd_p2g_term* terms,
+                                     int32_t halo_width, const cabana_amd_binning* bins,
+                                     cabana_amd_stream stream, void* scratch,
+                                     size_t scratch_bytes );
+// Records msg as the context's (and the thread's) last error; returns st.
+cabana_amd_status fail_with( cabana_amd_context* ctx, cabana_amd_status st, const char* msg );
 
 } // namespace cabana_amd

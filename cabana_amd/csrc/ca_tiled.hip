@@ -16,15 +16,16 @@
 //                         LDS.
 //   scatter_cells_kernel  P2G forward and the grid part of the G2P adjoint.
 //                         The part's particles are counting-sorted in LDS by
-//                         stencil base cell and stored there as compact
-//                         records (FP32/FP64 cell offsets + premultiplied
-//                         attributes, loaded with coalesced reads).  One lane
-//                         then owns a run of at most sub_len particles of one
-//                         base cell, accumulates their whole stencil in
-//                         registers from the LDS records only, and adds it
-//                         once into an FP64 LDS window (LDS FP64 adds are
-//                         ~16x faster than FP32 ones on gfx950).  The window
-//                         is flushed into the grid once per part.
+//                         stencil base cell (records: FP32/FP64 cell offsets
+//                         + premultiplied attributes, loaded with coalesced
+//                         reads a part ahead).  Each base cell's records form
+//                         tasks of at most SCATTER_TASK_MAX, ordered longest
+//                         first; a task's K*K lanes each own one (j, k) row of
+//                         the stencil, accumulate its K nodes for every term
+//                         in registers, and add them once into an FP64 LDS
+//                         window (LDS FP64 adds are ~16x faster than FP32 ones
+//                         on gfx950).  The window is flushed into the grid
+//                         once per part.
 #include "ca_device.hpp"
 
 #include <utility>
@@ -505,59 +506,6 @@ struct SlotDofs
 __device__ __forceinline__ float fma_r( float a, float b, float c ) { return fmaf( a, b, c ); }
 __device__ __forceinline__ double fma_r( double a, double b, double c ) { return fma( a, b, c ); }
 
-// Scatter contribution of one particle to the nodes of one x-slab (fixed
-// stencil index i, 1-D weight wx and gradient gx): with A = wy*wz,
-// B = gy*wz, C = wy*gz over the (j, k) nodes of the slab, the node weight is
-// W = wx*A and its gradient (gx*A, wx*B, wx*C), so every form's node value is
-// G[d] = A*u[d] + B*v[d] + C*w[d] for slab coefficients u, v, w formed once
-// per particle and slab (dB/dG of the forms in ca_device.hpp, regrouped).
-template <int FORM, class Real>
-__device__ __forceinline__ void slab_coeffs( const Real* P, Real wx, Real gx, Real* u,
-                                             Real* v, Real* w )
-{
-    if constexpr ( FORM == FORM_SV )
-    {
-        u[0] += wx * P[0];
-    }
-    else if constexpr ( FORM == FORM_VV )
-    {
-#pragma unroll
-        for ( int d = 0; d < 3; ++d )
-            u[d] += wx * P[d];
-    }
-    else if constexpr ( FORM == FORM_SG )
-    {
-        u[0] += gx * P[0];
-        v[1] += wx * P[0];
-        w[2] += wx * P[0];
-    }
-    else if constexpr ( FORM == FORM_VD )
-    {
-        u[0] += gx * P[0];
-        v[0] += wx * P[1];
-        w[0] += wx * P[2];
-    }
-    else if constexpr ( FORM == FORM_TD )
-    {
-#pragma unroll
-        for ( int a = 0; a < 3; ++a )
-        {
-            u[a] += gx * P[3 * a];
-            v[a] += wx * P[3 * a + 1];
-            w[a] += wx * P[3 * a + 2];
-        }
-    }
-    else
-    {
-#pragma unroll
-        for ( int b = 0; b < 3; ++b )
-        {
-            u[b] += gx * P[b];
-            v[b] += wx * P[3 + b];
-            w[b] += wx * P[6 + b];
-        }
-    }
-}
 
 // Record layout of a scatter pass: 3 fractional offsets, then the
 // attributes of every term in order.
@@ -575,201 +523,13 @@ struct RecordLayout
     static constexpr int len() { return attr( FL::N ); }
 };
 
-// Lanes per scatter task: the whole stencil (S = 1), one x-slab (S = K) or
-// one (x, y) pencil (S = K * K), the fewest that keep a lane's register
-// accumulator at <= SLAB_MAX_ACC values.
-constexpr int SLAB_MAX_ACC = 64;
-// Workgroup size of the scatter and its minimum waves per SIMD (the
-// register budget): two 256-thread workgroups per CU, so one workgroup's
-// memory phases overlap the other's accumulation.
-constexpr int SBLOCK = 256;
-constexpr int SCATTER_MIN_WAVES = 2;
-template <int DT>
-struct SlabPlan
-{
-    static constexpr int MAXACC = SLAB_MAX_ACC;
-    static constexpr int S =
-        ( K * K * K * DT <= MAXACC ) ? 1 : ( ( K * K * DT <= MAXACC ) ? K : K * K );
-    static constexpr int NI = ( S == 1 ) ? K : 1;
-    static constexpr int NJ = ( S == K * K ) ? 1 : K;
-};
-
-// Term groups of a scatter pass: each lane accumulates one group's terms.
-// Value-type terms (weights only) and gradient-type terms (weights and
-// gradients) form separate groups, so neither pays for the other's
-// products; terms that share one grid form a single group.
-template <bool SHARED, int... FORMS>
-struct TermGroups
-{
-    using FL = FormList<FORMS...>;
-    static constexpr bool has( bool grad )
-    {
-        for ( int t = 0; t < FL::N; ++t )
-            if ( form_value_type( FL::v[t] ) != grad )
-                return true;
-        return false;
-    }
-    static constexpr int N = ( SHARED || !has( false ) || !has( true ) ) ? 1 : 2;
-    // Group 1 (when there are two) holds the gradient-type terms.
-    static constexpr bool in( int t, int g )
-    {
-        return N == 1 ? true : ( form_value_type( FL::v[t] ) == ( g == 0 ) );
-    }
-    // Accumulator dofs of group g and term t's offset inside them.
-    static constexpr int dofs( int g )
-    {
-        if ( SHARED )
-            return 3;
-        int o = 0;
-        for ( int t = 0; t < FL::N; ++t )
-            if ( in( t, g ) )
-                o += ng_of( FL::v[t] );
-        return o;
-    }
-    static constexpr int offset( int t, int g )
-    {
-        if ( SHARED )
-            return 0;
-        int o = 0;
-        for ( int i = 0; i < t; ++i )
-            if ( in( i, g ) )
-                o += ng_of( FL::v[i] );
-        return o;
-    }
-    // True when accumulator dof d of group g receives a gradient-type term.
-    static constexpr bool grad_dof( int g, int d )
-    {
-        for ( int t = 0; t < FL::N; ++t )
-        {
-            if ( !in( t, g ) || form_value_type( FL::v[t] ) )
-                continue;
-            const int o = offset( t, g );
-            if ( SHARED || ( d >= o && d < o + ng_of( FL::v[t] ) ) )
-                return true;
-        }
-        return false;
-    }
-    static constexpr bool any_grad( int g )
-    {
-        for ( int t = 0; t < FL::N; ++t )
-            if ( in( t, g ) && !form_value_type( FL::v[t] ) )
-                return true;
-        return false;
-    }
-};
-
-// One record's contribution to a lane's slab accumulator for the terms of
-// group G.  The record holds the particle's three cell offsets and its
-// premultiplied attributes.
-template <class Real, bool SHARED, int G, int S, int... FORMS>
-__device__ __forceinline__ void scatter_record(
-    const Real* rp, const Real rdx[3], int sx, int sy,
-    Real ( *acc )[TermGroups<SHARED, FORMS...>::dofs( G )] )
-{
-    using FL = FormList<FORMS...>;
-    using TG = TermGroups<SHARED, FORMS...>;
-    using RL = RecordLayout<FORMS...>;
-    constexpr int NT = FL::N;
-    constexpr int DG = TG::dofs( G );
-    constexpr int NI = ( S == 1 ) ? K : 1;
-    constexpr int NJ = ( S == K * K ) ? 1 : K;
-    Spline1D<ORDER, Real> sp[3];
-#pragma unroll
-    for ( int d = 0; d < 3; ++d )
-        spline_weights<ORDER, Real, false>( rp[d], rdx[d], sp[d] );
-    Real wx[NI], gx[NI], wy[NJ], gy[NJ];
-    if constexpr ( S == 1 )
-    {
-#pragma unroll
-        for ( int i = 0; i < K; ++i )
-        {
-            wx[i] = sp[0].w[i];
-            gx[i] = sp[0].g[i];
-        }
-    }
-    else
-    {
-        wx[0] = sp[0].w[0];
-        gx[0] = sp[0].g[0];
-#pragma unroll
-        for ( int i = 1; i < K; ++i )
-        {
-            wx[0] = ( sx == i ) ? sp[0].w[i] : wx[0];
-            gx[0] = ( sx == i ) ? sp[0].g[i] : gx[0];
-        }
-    }
-    if constexpr ( S == K * K )
-    {
-        wy[0] = sp[1].w[0];
-        gy[0] = sp[1].g[0];
-#pragma unroll
-        for ( int j = 1; j < K; ++j )
-        {
-            wy[0] = ( sy == j ) ? sp[1].w[j] : wy[0];
-            gy[0] = ( sy == j ) ? sp[1].g[j] : gy[0];
-        }
-    }
-    else
-    {
-#pragma unroll
-        for ( int j = 0; j < K; ++j )
-        {
-            wy[j] = sp[1].w[j];
-            gy[j] = sp[1].g[j];
-        }
-    }
-#pragma unroll
-    for ( int i = 0; i < NI; ++i )
-    {
-        Real u[DG], v[DG], w[DG];
-#pragma unroll
-        for ( int d = 0; d < DG; ++d )
-            u[d] = v[d] = w[d] = Real( 0 );
-        static_for<NT>( [&]( auto I ) {
-            constexpr int t = I;
-            if constexpr ( TG::in( t, G ) )
-            {
-                constexpr int O = TG::offset( t, G );
-                slab_coeffs<FL::v[t]>( rp + RL::attr( t ), wx[i], gx[i], u + O, v + O, w + O );
-            }
-        } );
-        // Node (j, k) of the slab gets wy wz u + gy wz v + wy gz w
-        // = wz (wy u + gy v) + gz (wy w): per row j, P = wy u + gy v and
-        // Q = wy w (value-type dofs: P = wy u, Q = 0), then two FMAs per
-        // node and dof.
-#pragma unroll
-        for ( int j = 0; j < NJ; ++j )
-        {
-            Real P[DG], Q[DG];
-            static_for<DG>( [&]( auto D ) {
-                constexpr int d = D;
-                if constexpr ( TG::grad_dof( G, d ) )
-                {
-                    P[d] = fma_r( gy[j], v[d], wy[j] * u[d] );
-                    Q[d] = wy[j] * w[d];
-                }
-                else
-                    P[d] = wy[j] * u[d];
-            } );
-#pragma unroll
-            for ( int k = 0; k < K; ++k )
-                static_for<DG>( [&]( auto D ) {
-                    constexpr int d = D;
-                    Real& c = acc[( i * NJ + j ) * K + k][d];
-                    c = fma_r( sp[2].w[k], P[d], c );
-                    if constexpr ( TG::grad_dof( G, d ) )
-                        c = fma_r( sp[2].g[k], Q[d], c );
-                } );
-        }
-    }
-}
-
 // Task-table entry: base cell (x, y, z: 4 bits each), first record (15
-// bits), count (5).
+// bits), count - 1 (5 bits: 1 to SCATTER_TASK_MAX records).
+static_assert( SCATTER_TASK_MAX <= 32, "task entries hold count - 1 in 5 bits" );
 __device__ __forceinline__ unsigned pack_task( int lx, int ly, int lz, int start, int len )
 {
     return ( (unsigned)lx << 28 ) | ( (unsigned)ly << 24 ) | ( (unsigned)lz << 20 ) |
-           ( (unsigned)start << 5 ) | (unsigned)len;
+           ( (unsigned)start << 5 ) | (unsigned)( len - 1 );
 }
 
 // The run table of a brick in two halves, so its global reads can be issued
@@ -829,6 +589,193 @@ __device__ __forceinline__ void runs_commit( const RunLoad& rl, const TileGeom& 
         run_pre[0] = 0;
 }
 
+// One stencil node's weight polynomial (spline_coef / spline_shift of the
+// lane's node index), with the derivative's coefficients pre-scaled by rdx,
+// held per lane.
+template <class Real>
+struct NodePoly
+{
+    Real shift;
+    Real w[ORDER + 1];
+    Real g[ORDER];
+    __device__ __forceinline__ void init( int j, Real rdx )
+    {
+        shift = Real( spline_shift( ORDER, 0 ) );
+#pragma unroll
+        for ( int jj = 1; jj < K; ++jj )
+            shift = ( j == jj ) ? Real( spline_shift( ORDER, jj ) ) : shift;
+#pragma unroll
+        for ( int n = 0; n <= ORDER; ++n )
+        {
+            Real c = Real( spline_coef( ORDER, 0, n ) );
+#pragma unroll
+            for ( int jj = 1; jj < K; ++jj )
+                c = ( j == jj ) ? Real( spline_coef( ORDER, jj, n ) ) : c;
+            w[n] = c;
+        }
+#pragma unroll
+        for ( int n = 1; n <= ORDER; ++n )
+            g[n - 1] = Real( n ) * w[n] * rdx;
+    }
+    __device__ __forceinline__ void eval( Real f, Real& wv, Real& gv ) const
+    {
+        const Real t = f - shift;
+        wv = w[ORDER];
+#pragma unroll
+        for ( int n = ORDER - 1; n >= 0; --n )
+            wv = fma_r( wv, t, w[n] );
+        gv = g[ORDER - 1];
+#pragma unroll
+        for ( int n = ORDER - 2; n >= 0; --n )
+            gv = fma_r( gv, t, g[n] );
+    }
+};
+
+// Pencil form of a node value.  With A = wy wz, B = gy wz and C = wy gz of
+// a fixed (j, k) node row, the node at stencil index i along x has weight
+// wx_i A and weight gradient (gx_i A, wx_i B, wx_i C).  Every form's node
+// value (dB/dG of the forms in ca_device.hpp) is then
+//   G[d] = gx_i * alpha[d] + wx_i * beta[d]
+// with alpha and beta formed once per particle and row.  Value-type dofs
+// have no alpha; the y/z-gradient dofs of FORM_SG have no alpha either, and
+// its x-gradient dof no beta.
+template <int FORM>
+__host__ __device__ constexpr bool form_has_alpha( int d )
+{
+    return FORM == FORM_SG ? d == 0 : !( FORM == FORM_SV || FORM == FORM_VV );
+}
+template <int FORM>
+__host__ __device__ constexpr bool form_has_beta( int d )
+{
+    return FORM == FORM_SG ? d != 0 : true;
+}
+
+template <int FORM, class Real>
+__device__ __forceinline__ void pencil_coeffs( const Real* P, Real A, Real B, Real C,
+                                               Real* al, Real* be )
+{
+    if constexpr ( FORM == FORM_SV )
+    {
+        be[0] = fma_r( A, P[0], be[0] );
+    }
+    else if constexpr ( FORM == FORM_VV )
+    {
+#pragma unroll
+        for ( int d = 0; d < 3; ++d )
+            be[d] = fma_r( A, P[d], be[d] );
+    }
+    else if constexpr ( FORM == FORM_SG )
+    {
+        al[0] = fma_r( A, P[0], al[0] );
+        be[1] = fma_r( B, P[0], be[1] );
+        be[2] = fma_r( C, P[0], be[2] );
+    }
+    else if constexpr ( FORM == FORM_VD )
+    {
+        al[0] = fma_r( A, P[0], al[0] );
+        be[0] = fma_r( B, P[1], fma_r( C, P[2], be[0] ) );
+    }
+    else if constexpr ( FORM == FORM_TD )
+    {
+#pragma unroll
+        for ( int a = 0; a < 3; ++a )
+        {
+            al[a] = fma_r( A, P[3 * a], al[a] );
+            be[a] = fma_r( B, P[3 * a + 1], fma_r( C, P[3 * a + 2], be[a] ) );
+        }
+    }
+    else
+    {
+#pragma unroll
+        for ( int b = 0; b < 3; ++b )
+        {
+            al[b] = fma_r( A, P[b], al[b] );
+            be[b] = fma_r( B, P[3 + b], fma_r( C, P[6 + b], be[b] ) );
+        }
+    }
+}
+
+// Accumulator dofs of a scatter pass that receive an alpha / beta term.
+template <bool SHARED, int... FORMS>
+struct PencilDofs
+{
+    using FL = FormList<FORMS...>;
+    using SD = SlotDofs<FORMS...>;
+    template <int FORM>
+    static constexpr bool pick( int d, bool alpha )
+    {
+        return alpha ? form_has_alpha<FORM>( d ) : form_has_beta<FORM>( d );
+    }
+    static constexpr bool has( int d, bool alpha )
+    {
+        bool r = false;
+        int o = 0;
+        ( ( r = r || ( d >= ( SHARED ? 0 : o ) && d < ( SHARED ? 0 : o ) + ng_of( FORMS ) &&
+                       pick<FORMS>( d - ( SHARED ? 0 : o ), alpha ) ),
+            o += ng_of( FORMS ) ),
+          ... );
+        return r;
+    }
+};
+
+// One record's contribution to a lane's K x DT accumulator (nodes i of the
+// lane's (j, k) row).  The record holds the particle's three cell offsets
+// and its premultiplied attributes.
+template <class Real, bool SHARED, int... FORMS>
+__device__ __forceinline__ void pencil_record( const Real* rp, Real rdx0, const NodePoly<Real>& py,
+                                               const NodePoly<Real>& pz,
+                                               Real ( *acc )[SlotDofs<FORMS...>::total( SHARED )] )
+{
+    using FL = FormList<FORMS...>;
+    using SD = SlotDofs<FORMS...>;
+    using RL = RecordLayout<FORMS...>;
+    using PD = PencilDofs<SHARED, FORMS...>;
+    constexpr int DT = SD::total( SHARED );
+    Spline1D<ORDER, Real> sx;
+    spline_weights<ORDER, Real, false>( rp[0], rdx0, sx );
+    const Real* wx = sx.w;
+    const Real* gx = sx.g;
+    Real wy, gy, wz, gz;
+    py.eval( rp[1], wy, gy );
+    pz.eval( rp[2], wz, gz );
+    const Real A = wy * wz, B = gy * wz, C = wy * gz;
+    Real al[DT], be[DT];
+#pragma unroll
+    for ( int d = 0; d < DT; ++d )
+        al[d] = be[d] = Real( 0 );
+    static_for<FL::N>( [&]( auto I ) {
+        constexpr int t = I;
+        constexpr int O = SD::offset( t, SHARED );
+        pencil_coeffs<FL::v[t]>( rp + RL::attr( t ), A, B, C, al + O, be + O );
+    } );
+#pragma unroll
+    for ( int i = 0; i < K; ++i )
+        static_for<DT>( [&]( auto D ) {
+            constexpr int d = D;
+            if constexpr ( PD::has( d, true ) )
+                acc[i][d] = fma_r( gx[i], al[d], acc[i][d] );
+            if constexpr ( PD::has( d, false ) )
+                acc[i][d] = fma_r( wx[i], be[d], acc[i][d] );
+        } );
+}
+
+// Reads one LDS record (16-byte aligned) into registers.
+template <class Real, int N>
+__device__ __forceinline__ void read_record( const Real* rp, Real* v )
+{
+    constexpr int W = 16 / (int)sizeof( Real );
+    typedef Real V __attribute__( ( ext_vector_type( W ) ) );
+#pragma unroll
+    for ( int c = 0; c < N; c += W )
+    {
+        const V x = *reinterpret_cast<const V*>( rp + c );
+#pragma unroll
+        for ( int e = 0; e < W; ++e )
+            if ( c + e < N )
+                v[c + e] = x[e];
+    }
+}
+
 // Particles of one part held in registers between their loads and their use
 // (the loads of part k + 1 are in flight while part k accumulates).
 template <class Real, int MAXM, int NPT>
@@ -853,7 +800,6 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
 #pragma unroll
     for ( int mi = 0; mi < MAXM; ++mi )
         run[mi] = 0;
-#pragma unroll 8
     for ( int q = 1; q < nruns; ++q )
     {
         const int rp = run_pre[q];
@@ -879,8 +825,23 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
     }
 }
 
+// Workgroup size of the scatter and its minimum waves per SIMD (the
+// register budget): three 256-thread workgroups per CU for FP32 records,
+// two for FP64 ones (whose accumulators and records take twice the
+// registers).
+template <class Real>
+constexpr int sblock()
+{
+    return scatter_block( sizeof( Real ) == 8 );
+}
+template <class Real>
+constexpr int scatter_min_waves()
+{
+    return sizeof( Real ) == 4 ? 3 : 2;
+}
+
 template <class Real, bool CONTIG, bool SHARED, int... FORMS>
-__global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_kernel(
+__global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void scatter_cells_kernel(
     ScatterArgs a, TileArgs ta )
 {
     using FL = FormList<FORMS...>;
@@ -888,13 +849,21 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
     using RL = RecordLayout<FORMS...>;
     constexpr int NT = FL::N;
     constexpr int DT = SD::total( SHARED );
-    constexpr int RS = record_stride( RL::len(), (int)sizeof( Real ) );
-    constexpr int MAXM = ( SCATTER_CAP_MAX + SBLOCK - 1 ) / SBLOCK;
-    constexpr int NPT = RL::len() - 3;
+    constexpr int RLEN = RL::len();
+    constexpr int RS = record_stride( RLEN, (int)sizeof( Real ) );
+    constexpr int SBLOCK = sblock<Real>();
+    constexpr int MAXM = SCATTER_SLOTS;
+    constexpr int NPT = RLEN - 3;
+    constexpr int LMAX = SCATTER_TASK_MAX;
+    // Pencil lanes: a task's K*K lanes own the (j, k) rows of its stencil.
+    constexpr int KK = K * K;
+    constexpr int GPW = 64 / KK;
+    constexpr int NGROUP = SBLOCK / 64 * GPW;
     extern __shared__ __align__( 16 ) unsigned char smem[];
 
     // LDS layout (capacities fixed on the host for a full brick): FP64
-    // window [node][DT], base-cell offsets, task table, records, two run
+    // window [node][DT], base-cell offsets, task table (sorted, unsorted,
+    // length slots, length histogram), sorted record order, records, two run
     // tables, part-table entries, scan scratch.
     double* win = reinterpret_cast<double*>( smem );
     size_t off = align16( sizeof( double ) * (size_t)DT * ta.max_win_nodes );
@@ -902,6 +871,15 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
     off = align16( off + sizeof( int ) * ( ta.max_base_cells + 1 ) );
     unsigned* task = reinterpret_cast<unsigned*>( smem + off );
     off = align16( off + sizeof( unsigned ) * ta.max_tasks );
+    unsigned* task_in = reinterpret_cast<unsigned*>( smem + off );
+    off = align16( off + sizeof( unsigned ) * ta.max_tasks );
+    int* task_slot = reinterpret_cast<int*>( smem + off );
+    off = align16( off + sizeof( int ) * ta.max_tasks );
+    int* len_hist = reinterpret_cast<int*>( smem + off );
+    int* len_off = len_hist + LMAX;
+    off = align16( off + sizeof( int ) * 2 * LMAX );
+    uint16_t* ord = reinterpret_cast<uint16_t*>( smem + off );
+    off = align16( off + sizeof( uint16_t ) * ta.cap );
     Real* rec = reinterpret_cast<Real*>( smem + off );
     off = align16( off + sizeof( Real ) * (size_t)RS * ta.cap );
     int64_t* run_start[2];
@@ -917,8 +895,6 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
     off = align16( off + sizeof( int2 ) * 128 );
     int* wsum = reinterpret_cast<int*>( smem + off );
 
-    const Real rdx[3] = { (Real)a.mesh.rdx[0], (Real)a.mesh.rdx[1], (Real)a.mesh.rdx[2] };
-    const int L = ta.sub_len;
     const int W[3] = { ta.win[0], ta.win[1], ta.win[2] };
     const int wnodes = W[0] * W[1] * W[2];
     const float rW[3] = { 1.0f / (float)W[0], 1.0f / (float)W[1], 1.0f / (float)W[2] };
@@ -938,16 +914,29 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
         }
     };
     int loc = pl.first;
-    int w = part_at( loc );
-    if ( w < 0 )
+    if ( part_at( loc ) < 0 )
         return;
-    int kpart = 0;
+
+    // The lane's task group and stencil row, and the weight polynomials of
+    // its row's y and z nodes.
+    const int lane = (int)threadIdx.x & 63;
+    const int gw = lane / KK;
+    const int lg = lane - gw * KK;
+    const int pj = lg / K, pk = lg - ( lg / K ) * K;
+    const bool pencil_lane = gw < GPW;
+    const int gid = ( (int)threadIdx.x >> 6 ) * GPW + gw;
+    const Real rdx0 = (Real)a.mesh.rdx[0];
+    NodePoly<Real> py, pz;
+    py.init( pj, (Real)a.mesh.rdx[1] );
+    pz.init( pk, (Real)a.mesh.rdx[2] );
+    const int wstride_i = W[1] * W[2] * DT;
 
     // Prologue: the first part's run table and particle loads; zeroed
     // window and counts; the first 64 part-table entries.
-    TileGeom tg;
-    part_geometry( ta, a.mesh, w, tg );
     load_parts( 0 );
+    TileGeom tg;
+    part_geometry_of( ta, a.mesh, ta.parts[part_at( loc )], tg );
+    if ( threadIdx.x < 64 )
     {
         RunLoad rl;
         runs_issue( ta, a.mesh, tg, rl );
@@ -957,25 +946,26 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
         win[e] = 0.0;
     for ( int c = threadIdx.x; c <= ta.max_base_cells; c += SBLOCK )
         cnt[c] = 0;
+    if ( threadIdx.x < LMAX )
+        len_hist[threadIdx.x] = 0;
     __syncthreads();
     int buf = 0;
     PartPrefetch<Real, MAXM, NPT> pf;
     {
         const int nruns = ( tg.c1[0] - tg.c0[0] ) * ( tg.c1[1] - tg.c0[1] );
-        const int total = run_pre[0][nruns];
         const int pbeg = tg.part * ta.cap;
-        prefetch_part<Real, CONTIG, MAXM, SBLOCK, FORMS...>( a, ta, run_start[0], run_pre[0], nruns,
-                                                     pbeg, min( ta.cap, total - pbeg ), pf );
+        prefetch_part<Real, CONTIG, MAXM, SBLOCK, FORMS...>(
+            a, ta, run_start[0], run_pre[0], nruns, pbeg,
+            min( ta.cap, run_pre[0][nruns] - pbeg ), pf );
     }
-
-    for ( ;; )
+    for ( int kpart = 0;; ++kpart )
     {
         // The next part: its run table is read now and committed after B.
         const int loc1 = loc + pl.stride;
-        const int w1 = part_at( loc1 );
+        const bool more = part_at( loc1 ) >= 0;
         TileGeom tg1 = tg;
         RunLoad rl1 = { 0, 0, 0 };
-        if ( w1 >= 0 )
+        if ( more )
             part_geometry_of( ta, a.mesh, ptab[( kpart + 1 ) & 127], tg1 );
         // Base cells of the brick and the node window they reach.
         int blo[3], NB[3];
@@ -987,27 +977,29 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
         }
         const int nbt = NB[0] * NB[1] * NB[2];
 
-        // A: stencil base cell of every particle of the part (from the
-        // prefetched registers), counted per base cell (the slot is the
-        // particle's rank within its base cell); the record (offsets,
-        // premultiplied attributes) stays in registers until its place in
-        // the sorted order is known.
+        // A: every particle of the part (loaded into registers during the
+        // previous part) gets its stencil base cell, counted per base cell
+        // (the slot is the particle's rank within it), and its record (cell
+        // offsets, premultiplied attributes) is written to LDS at its part
+        // index.
         int key[MAXM];
-        Real stash[MAXM][RL::len()];
 #pragma unroll
         for ( int mi = 0; mi < MAXM; ++mi )
         {
             key[mi] = -1;
             if ( pf.p[mi] < 0 )
                 continue;
-            const int64_t p = pf.p[mi];
+            const int i = (int)threadIdx.x + mi * SBLOCK;
+            // key: -1 no particle or outside the mesh, -2 outside the brick,
+            // else base cell << 16 | rank.
             int bs[3];
+            Real v[RLEN];
             bool ok = true;
 #pragma unroll
             for ( int d = 0; d < 3; ++d )
-                ok = spline_frac<ORDER, Real>(
-                         logical_coord<ORDER>( pf.x[mi][d], a.mesh.low[d], a.mesh.rdx[d] ),
-                         a.mesh.num_node[d], bs[d], stash[mi][d] ) &&
+                ok = spline_frac<ORDER, Real>( logical_coord<ORDER>( pf.x[mi][d], a.mesh.low[d],
+                                                                     a.mesh.rdx[d] ),
+                                               a.mesh.num_node[d], bs[d], v[d] ) &&
                      ok;
             if ( !ok )
             {
@@ -1032,29 +1024,39 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
                     const Real m = (Real)a.t[t].mult;
 #pragma unroll
                     for ( int c = 0; c < NP; ++c )
-                        stash[mi][RL::attr( t ) + c] = pf.attr[mi][RL::attr( t ) - 3 + c] * m;
+                        v[RL::attr( t ) + c] = pf.attr[mi][RL::attr( t ) - 3 + c] * m;
                 } );
+                detail::vec_copy_out<Real, RLEN>( v, rec + (size_t)RS * i );
             }
             else
             {
-                // Not sorted into this brick: add straight to the grid.
-                Spline3D<ORDER, Real, false> sp;
-                eval_spline( a.mesh, pf.x[mi], sp );
-                global_scatter_particle<Real, SHARED, FORMS...>( a, p, sp );
+                key[mi] = -2;
             }
         }
+        // Particles not sorted into this brick (a binning older than the
+        // positions) are added straight to the grid, after the common path
+        // so that its registers are free.
+#pragma unroll
+        for ( int mi = 0; mi < MAXM; ++mi )
+            if ( key[mi] == -2 )
+            {
+                Spline3D<ORDER, Real, false> sp;
+                eval_spline( a.mesh, pf.x[mi], sp );
+                global_scatter_particle<Real, SHARED, FORMS...>( a, pf.p[mi], sp );
+            }
         // The next part's run table reads, committed after B.
-        if ( w1 >= 0 )
+        if ( more )
             runs_issue( ta, a.mesh, tg1, rl1 );
         __syncthreads();
         // B: base-cell offsets (cnt[nbt] = sorted particles), then the task
-        // table: every base cell's c records cut into ceil(c / L) runs of
-        // near-equal length (at most L; a wave waits for its longest run).
-        // One scan of (count | tasks << 16) per base cell gives both the
-        // record offsets (low half) and the task offsets (high half); the
-        // halves cannot carry into each other (a part holds < 2^15 records).
+        // table: every base cell's c records cut into ceil(c / LMAX) runs of
+        // near-equal length.  One scan of (count | tasks << 16) per base cell
+        // gives both the record offsets (low half) and the task offsets (high
+        // half); the halves cannot carry into each other (a part holds < 2^15
+        // records).  Each task also takes a slot in a histogram of task
+        // lengths (longest first), by which it is placed below.
         const int packed_total = block_exclusive_scan<SBLOCK>(
-            cnt, nbt, wsum, [L]( int c ) { return c | ( ( c + L - 1 ) / L ) << 16; } );
+            cnt, nbt, wsum, []( int c ) { return c | ( ( c + LMAX - 1 ) / LMAX ) << 16; } );
         const int ntask = packed_total >> 16;
         for ( int b = threadIdx.x; b < nbt; b += SBLOCK )
         {
@@ -1065,126 +1067,103 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
             const int ly = ( b / NB[2] ) % NB[1];
             const int lz = b % NB[2];
             int q = e0 >> 16;
-            const int nt = ( c + L - 1 ) / L;
+            const int nt = ( c + LMAX - 1 ) / LMAX;
             const int len = nt > 0 ? c / nt : 0, longer = nt > 0 ? c - len * nt : 0;
             for ( int i = 0, r = 0; i < nt; ++i )
             {
                 const int n = len + ( i < longer ? 1 : 0 );
-                task[q++] = pack_task( lx, ly, lz, c0 + r, n );
+                const int bucket = LMAX - n;
+                task_in[q] = pack_task( lx, ly, lz, c0 + r, n );
+                task_slot[q] = ( bucket << 16 ) | atomicAdd( len_hist + bucket, 1 );
+                ++q;
                 r += n;
             }
         }
-        if ( w1 >= 0 )
+        if ( more )
             runs_commit( rl1, tg1, run_start[buf ^ 1], run_pre[buf ^ 1] );
-        // C: records in base-cell order.
+        // C: the sorted order of the records (base cell, then rank).
 #pragma unroll
         for ( int mi = 0; mi < MAXM; ++mi )
+            if ( key[mi] >= 0 )
+                ord[( cnt[key[mi] >> 16] & 0xffff ) + ( key[mi] & 0xffff )] =
+                    (uint16_t)( threadIdx.x + mi * SBLOCK );
+        __syncthreads();
+        // The task table in order of decreasing length: offsets of the length
+        // buckets (one wave; the histogram is cleared for the next part),
+        // then every task to its bucket's offset plus its slot.
+        if ( threadIdx.x < 64 )
         {
-            if ( key[mi] < 0 )
-                continue;
-            Real* rp = rec + (size_t)RS * ( ( cnt[key[mi] >> 16] & 0xffff ) + ( key[mi] & 0xffff ) );
+            const int lane = threadIdx.x;
+            const int h = lane < LMAX ? len_hist[lane] : 0;
+            int v = h;
 #pragma unroll
-            for ( int c = 0; c < RL::len(); ++c )
-                rp[c] = stash[mi][c];
+            for ( int o = 1; o < 64; o <<= 1 )
+            {
+                const int u = __shfl_up( v, o, 64 );
+                if ( lane >= o )
+                    v += u;
+            }
+            if ( lane < LMAX )
+            {
+                len_off[lane] = v - h;
+                len_hist[lane] = 0;
+            }
+        }
+        __syncthreads();
+        for ( int q = threadIdx.x; q < ntask; q += SBLOCK )
+        {
+            const int sl = task_slot[q];
+            task[len_off[sl >> 16] + ( sl & 0xffff )] = task_in[q];
         }
         __syncthreads();
         // The next part's particle reads are issued here and complete while
         // this part accumulates.
-        if ( w1 >= 0 )
+        if ( more )
         {
             const int nruns1 = ( tg1.c1[0] - tg1.c0[0] ) * ( tg1.c1[1] - tg1.c0[1] );
-            const int total1 = run_pre[buf ^ 1][nruns1];
             const int pbeg1 = tg1.part * ta.cap;
-            prefetch_part<Real, CONTIG, MAXM, SBLOCK, FORMS...>( a, ta, run_start[buf ^ 1],
-                                                         run_pre[buf ^ 1], nruns1, pbeg1,
-                                                         min( ta.cap, total1 - pbeg1 ), pf );
+            prefetch_part<Real, CONTIG, MAXM, SBLOCK, FORMS...>(
+                a, ta, run_start[buf ^ 1], run_pre[buf ^ 1], nruns1, pbeg1,
+                min( ta.cap, run_pre[buf ^ 1][nruns1] - pbeg1 ), pf );
         }
-        // D: one lane per (task, term group, stencil slab).  The lane walks
-        // its task's records, accumulates its group's node values over its
-        // slab in registers, and adds them once into the window.  Each
-        // group's lanes start on a wave boundary, so a wave runs one group's
-        // code; the lanes of one task read the same records (broadcast).
+        // D: a task's K*K lanes each own one (j, k) row of the stencil: they
+        // walk the task's records (the same record across the group: an
+        // LDS broadcast), accumulate the K nodes of their row in registers
+        // for every term, and add them once into the FP64 window.
+        // Tasks go to the groups in rounds, longest first, alternately
+        // forwards and backwards over the groups, so the groups' total
+        // lengths stay close.
+        if ( pencil_lane )
         {
-            using TGR = TermGroups<SHARED, FORMS...>;
-            constexpr int NGR = TGR::N;
-            constexpr int RLEN = RL::len();
-            int seg[NGR + 1];
-            seg[0] = 0;
-            static_for<NGR>( [&]( auto GI ) {
-                constexpr int grp = GI;
-                constexpr int S = SlabPlan<TGR::dofs( grp )>::S;
-                seg[grp + 1] = seg[grp] + ( ntask * S + 63 ) / 64 * 64;
-            } );
-            for ( int tt = threadIdx.x; tt < seg[NGR]; tt += SBLOCK )
+            for ( int round = 0;; ++round )
             {
-                static_for<NGR>( [&]( auto GI ) {
-                    constexpr int grp = GI;
-                    constexpr int DG = TGR::dofs( grp );
-                    using SP = SlabPlan<DG>;
-                    constexpr int S = SP::S;
-                    constexpr int NI = SP::NI;
-                    constexpr int NJ = SP::NJ;
-                    constexpr int NN = NI * NJ * K;
-                    const int local = tt - seg[grp];
-                    if ( local < 0 || local >= ntask * S )
-                        return;
-                    const int q = ( S == 1 ) ? local : local / S;
-                    const int sa = ( S == 1 ) ? 0 : local - q * S;
-                    const int sx = ( S == 1 ) ? 0 : ( S == K ? sa : sa / K );
-                    const int sy = ( S == K * K ) ? sa - sx * K : 0;
-                    const unsigned e = task[q];
-                    const int lx = (int)( e >> 28 );
-                    const int ly = (int)( ( e >> 24 ) & 15 );
-                    const int lz = (int)( ( e >> 20 ) & 15 );
-                    const int r0 = (int)( ( e >> 5 ) & 0x7fff );
-                    const int r1 = r0 + (int)( e & 31 );
-                    Real acc[NN][DG];
+                const int q = round * NGROUP + ( ( round & 1 ) ? NGROUP - 1 - gid : gid );
+                if ( q >= ntask )
+                    break;
+                const unsigned e = task[q];
+                const int lx = (int)( e >> 28 );
+                const int ly = (int)( ( e >> 24 ) & 15 );
+                const int lz = (int)( ( e >> 20 ) & 15 );
+                const int r0 = (int)( ( e >> 5 ) & 0x7fff );
+                const int n = (int)( e & 31 ) + 1;
+                Real acc[K][DT];
 #pragma unroll
-                    for ( int n = 0; n < NN; ++n )
+                for ( int i = 0; i < K; ++i )
 #pragma unroll
-                        for ( int d = 0; d < DG; ++d )
-                            acc[n][d] = Real( 0 );
+                    for ( int d = 0; d < DT; ++d )
+                        acc[i][d] = Real( 0 );
+                for ( int r = r0; r < r0 + n; ++r )
+                {
                     Real cur[RLEN];
+                    read_record<Real, RLEN>( rec + (size_t)RS * ord[r], cur );
+                    pencil_record<Real, SHARED, FORMS...>( cur, rdx0, py, pz, acc );
+                }
+                double* wb = win + ( ( lx * W[1] + ly + pj ) * W[2] + lz + pk ) * DT;
 #pragma unroll
-                    for ( int c = 0; c < RLEN; ++c )
-                        cur[c] = rec[(size_t)RS * r0 + c];
-                    for ( int r = r0; r < r1; ++r )
-                    {
-                        const int rn = min( r + 1, r1 - 1 );
-                        Real nxt[RLEN];
+                for ( int i = 0; i < K; ++i )
 #pragma unroll
-                        for ( int c = 0; c < RLEN; ++c )
-                            nxt[c] = rec[(size_t)RS * rn + c];
-                        scatter_record<Real, SHARED, grp, S, FORMS...>( cur, rdx, sx, sy, acc );
-#pragma unroll
-                        for ( int c = 0; c < RLEN; ++c )
-                            cur[c] = nxt[c];
-                    }
-                    double* wb = win + ( ( lx + sx ) * W[1] + ly + sy ) * W[2] * DT + lz * DT;
-#pragma unroll
-                    for ( int i = 0; i < NI; ++i )
-#pragma unroll
-                        for ( int j = 0; j < NJ; ++j )
-#pragma unroll
-                            for ( int k = 0; k < K; ++k )
-                            {
-                                double* q2 = wb + ( ( i * W[1] + j ) * W[2] + k ) * DT;
-                                static_for<NT>( [&]( auto I ) {
-                                    constexpr int t = I;
-                                    if constexpr ( TGR::in( t, grp ) &&
-                                                   ( !SHARED || t == 0 ) )
-                                    {
-                                        constexpr int NGT = SHARED ? 3 : ng_of( FL::v[t] );
-                                        constexpr int OW = SD::offset( t, SHARED );
-                                        constexpr int OA = TGR::offset( t, grp );
-#pragma unroll
-                                        for ( int d = 0; d < NGT; ++d )
-                                            atomicAdd( q2 + OW + d,
-                                                       (double)acc[( i * NJ + j ) * K + k][OA + d] );
-                                    }
-                                } );
-                            }
-                } );
+                    for ( int d = 0; d < DT; ++d )
+                        atomicAdd( wb + i * wstride_i + d, (double)acc[i][d] );
             }
         }
         __syncthreads();
@@ -1230,13 +1209,11 @@ __global__ __launch_bounds__( SBLOCK, SCATTER_MIN_WAVES ) void scatter_cells_ker
             load_parts( kpart + 2 );
         // The next part reuses the window, the counts and the tables.
         __syncthreads();
-        if ( w1 < 0 )
+        if ( !more )
             break;
         loc = loc1;
-        w = w1;
         tg = tg1;
         buf ^= 1;
-        ++kpart;
     }
 }
 
@@ -1328,7 +1305,7 @@ hipError_t scatter_cells_launch( int plan, const ScatterArgs& a, const TileArgs&
         {
 #define CA_CASE( F )                                                           \
     case F:                                                                    \
-        return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, false, F>, a, t, lds, s, SBLOCK );
+        return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, false, F>, a, t, lds, s, sblock<Real>() );
             CA_CASE( FORM_SV )
             CA_CASE( FORM_VV )
             CA_CASE( FORM_SG )
@@ -1343,13 +1320,13 @@ hipError_t scatter_cells_launch( int plan, const ScatterArgs& a, const TileArgs&
     if ( plan == PLAN_MPM_P2G )
         return launch_tile_kernel(
             scatter_cells_kernel<Real, CONTIG, false, FORM_SV, FORM_VV, FORM_TD>, a, t, lds, s,
-            SBLOCK );
+            sblock<Real>() );
     if ( plan == PLAN_MASS_MOMENTUM )
         return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, false, FORM_SV, FORM_VV>,
-                                   a, t, lds, s, SBLOCK );
+                                   a, t, lds, s, sblock<Real>() );
     if ( plan == PLAN_MPM_G2P )
         return launch_tile_kernel( scatter_cells_kernel<Real, CONTIG, true, FORM_VV, FORM_TDT>,
-                                   a, t, lds, s, SBLOCK );
+                                   a, t, lds, s, sblock<Real>() );
     return hipErrorInvalidValue;
 }
 

@@ -268,8 +268,8 @@ def test_config5_full_shape(ca):
     # (2) one step, FP32-mixed against FP64.
     res = {}
     for dt in (torch.float64, torch.float32):
-        xr = x0.clone().requires_grad_()
-        vr = v0.to(dt).requires_grad_()
+        xr = x0.detach().clone().requires_grad_()
+        vr = v0.detach().to(dt).clone().requires_grad_()
         rec = []
         x1, v1, _, gv = mpm.explicit_step(box, xr, vr, m0.to(dt), record=rec)
         loss = (v1.double() ** 2).sum() + 1e-3 * x1.sum() + 1e-2 * gv.double().sum()
@@ -290,8 +290,8 @@ def test_config5_full_shape(ca):
     # (3) checkpointed against plain, 4 steps.
     grads = []
     for every in (0, 2):
-        xr = x0.clone().requires_grad_()
-        vr = v0.float().requires_grad_()
+        xr = x0.detach().clone().requires_grad_()
+        vr = v0.detach().float().clone().requires_grad_()
         xs, vs, _, _ = mpm.run_loop(box, xr, vr, m32, num_steps=4, checkpoint_every=every)
         ((vs.double() ** 2).sum() + 1e-3 * xs.sum()).backward()
         grads.append((xr.grad.double(), vr.grad.double()))
