@@ -61,8 +61,9 @@ def build_library(verbose=False):
                                             os.path.join(CSRC, src + ".hip"), "-o", obj])
     jobs.append([hipcc] + COMMON + ["-c", os.path.join(CSRC, "ca_grid.hip"), "-o",
                                     os.path.join(BUILD, "ca_grid.o")])
-    jobs.append([hipcc] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, "ca_api.cpp"),
-                                    "-o", os.path.join(BUILD, "ca_api.o")])
+    for src in ("ca_api", "ca_no_alloc"):
+        jobs.append([hipcc] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src + ".cpp"),
+                                        "-o", os.path.join(BUILD, src + ".o")])
     workers = max(1, min(len(jobs), os.cpu_count() or 1))
     with ThreadPoolExecutor(max_workers=workers) as ex:
         outs = list(ex.map(_run, jobs))

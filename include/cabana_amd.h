@@ -219,6 +219,29 @@ cabana_amd_status cabana_amd_p2g( cabana_amd_context* ctx,
                                   const cabana_amd_binning* bins,
                                   cabana_amd_stream stream );
 
+/* No reference counterpart: cabana_amd_p2g for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, semantics and results
+ * as cabana_amd_p2g; the call allocates nothing (no hipMalloc/hipFree, no
+ * part-table cache, kernel timing off).  The brick part table of a binned
+ * (tiled) call is built in a fixed device array of
+ * CABANA_AMD_NO_ALLOC_SCRATCH_BYTES per device; a call whose table would
+ * not fit returns CABANA_AMD_ERROR_INVALID_ARGUMENT (about 1.3 M bricks plus
+ * parts: a 256^3 mesh at any density fits).  Every call on one device uses
+ * that device's array: calls that overlap on one device (several streams or
+ * host threads) race and must be serialised by the caller.  The context's
+ * device must be current. */
+#define CABANA_AMD_NO_ALLOC_SCRATCH_BYTES ( (size_t)32 << 20 )
+cabana_amd_status cabana_amd_p2g_no_alloc( cabana_amd_context* ctx,
+                                           const cabana_amd_mesh* mesh,
+                                           int32_t spline_order,
+                                           const cabana_amd_particle_field* points,
+                                           int32_t num_terms,
+                                           const cabana_amd_p2g_term* terms,
+                                           int32_t halo_width,
+                                           const cabana_amd_binning* bins,
+                                           cabana_amd_stream stream );
+
 /* g2p(array, halo, points, num_point, Spline<order>, functor)
  * (Cabana_Grid_Interpolation.hpp:831-927) for 1 to 16 functors.  Every
  * distinct grid first gets its ghosts filled from their periodic owners
