@@ -54,8 +54,10 @@ def build_library(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     hipcc = _hipcc()
     jobs = []
-    for order in (1, 2, 3):
-        for src in ("ca_kernels", "ca_tiled"):
+    # Per-particle kernels for Spline<0> .. Spline<6>; cell-tiled kernels for
+    # orders 1 to 3.
+    for order in range(7):
+        for src in ("ca_kernels", "ca_tiled") if 1 <= order <= 3 else ("ca_kernels",):
             obj = os.path.join(BUILD, "%s_o%d.o" % (src, order))
             jobs.append([hipcc] + COMMON + ["-DCA_ORDER=%d" % order, "-c",
                                             os.path.join(CSRC, src + ".hip"), "-o", obj])

@@ -425,9 +425,9 @@ bool same_grid( const GFieldDev& a, const GFieldDev& b )
 
 cabana_amd_status check_order( cabana_amd_context* ctx, int order )
 {
-    if ( order < 1 || order > 3 )
+    if ( order < 0 || order > 6 )
         return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED,
-                     "spline order must be 1, 2 or 3" );
+                     "spline order must be 0 to 6" );
     return CABANA_AMD_OK;
 }
 
@@ -461,12 +461,20 @@ hipError_t run_gather( cabana_amd_context* ctx, int order, bool fp64,
 {
     switch ( order )
     {
+    case 0:
+        return launch_gather_o0( *ctx, fp64, plan, a, s );
     case 1:
         return launch_gather_o1( *ctx, fp64, plan, a, s );
     case 2:
         return launch_gather_o2( *ctx, fp64, plan, a, s );
-    default:
+    case 3:
         return launch_gather_o3( *ctx, fp64, plan, a, s );
+    case 4:
+        return launch_gather_o4( *ctx, fp64, plan, a, s );
+    case 5:
+        return launch_gather_o5( *ctx, fp64, plan, a, s );
+    default:
+        return launch_gather_o6( *ctx, fp64, plan, a, s );
     }
 }
 
@@ -475,12 +483,20 @@ hipError_t run_scatter_atomic( cabana_amd_context* ctx, int order, bool fp64,
 {
     switch ( order )
     {
+    case 0:
+        return launch_scatter_atomic_o0( *ctx, fp64, plan, a, s );
     case 1:
         return launch_scatter_atomic_o1( *ctx, fp64, plan, a, s );
     case 2:
         return launch_scatter_atomic_o2( *ctx, fp64, plan, a, s );
-    default:
+    case 3:
         return launch_scatter_atomic_o3( *ctx, fp64, plan, a, s );
+    case 4:
+        return launch_scatter_atomic_o4( *ctx, fp64, plan, a, s );
+    case 5:
+        return launch_scatter_atomic_o5( *ctx, fp64, plan, a, s );
+    default:
+        return launch_scatter_atomic_o6( *ctx, fp64, plan, a, s );
     }
 }
 
@@ -667,12 +683,15 @@ bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int 
 }
 
 // Tile parameters for cell-sorted particles; false when the binning does not
-// describe the mesh's own cells.
+// describe the mesh's own cells or the order has no tiled kernels (only
+// orders 1 to 3 do).
 bool make_tiles( TileKind kind, const MeshDev& m, int order, bool fp64,
                  const int* slot_ng, int nslot, int rec_len,
                  const cabana_amd_binning* bins, const GFieldDev& g0,
                  TileArgs& t, size_t& lds )
 {
+    if ( order < 1 || order > 3 )
+        return false;
     if ( !bins || !bins->counts || !bins->offsets )
         return false;
     for ( int d = 0; d < 3; ++d )

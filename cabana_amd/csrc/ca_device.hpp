@@ -23,8 +23,8 @@ __device__ __forceinline__ void static_for( F&& f )
 }
 
 // Logical coordinate of a position, Spline<N>::mapToLogicalGrid
-// (Cabana_Grid_Splines.hpp :141-145 primal for odd orders, :239-244 dual for
-// even ones).  Rounded exactly as written: the cell index derived from it
+// (Cabana_Grid_Splines.hpp :146-149, :352-355, :622-625 primal for odd
+// orders, :61-64, :241-244, :478-481, :785-788 dual for even ones).  Rounded exactly as written: the cell index derived from it
 // must not depend on FMA contraction.
 template <int ORDER>
 __device__ __forceinline__ double logical_coord( double xp, double low,
@@ -32,7 +32,7 @@ __device__ __forceinline__ double logical_coord( double xp, double low,
 {
 #pragma clang fp contract( off )
     double x = ( xp - low ) * rdx;
-    if ( ORDER == 0 || ORDER == 2 )
+    if ( ORDER % 2 == 0 )
         x = x + 0.5;
     return x;
 }
@@ -50,8 +50,9 @@ struct Spline1D
 };
 
 // Stencil base node and fractional offset of a logical coordinate
-// (Spline<ORDER>::stencil, Cabana_Grid_Splines.hpp <1> :163-168, <2> :264-270,
-// <3> :383-390).  The offset x0 - int(x0) is formed in FP64 and only then
+// (Spline<ORDER>::stencil, Cabana_Grid_Splines.hpp <0> :80-84, <1> :169-175,
+// <2> :265-272, <3> :377-385, <4> :504-512, <5> :649-659, <6> :813-824: the
+// first node is int(x0) - ORDER / 2).  The offset x0 - int(x0) is formed in FP64 and only then
 // rounded to Real, so the FP32 path keeps FP64 cell indices and offsets.
 // Returns false when the stencil leaves [0, num_node) or the coordinate is
 // not finite.
@@ -64,7 +65,7 @@ __device__ __forceinline__ bool spline_frac( double x0, int num_node, int& base,
     if ( !( x0 >= 0.0 && x0 < (double)num_node ) )
         return false;
     const int i0 = (int)x0;
-    base = ( ORDER >= 2 ) ? i0 - 1 : i0;
+    base = i0 - ORDER / 2;
     if ( base < 0 || base + K > num_node )
         return false;
     f = (Real)( x0 - (double)i0 );
@@ -73,34 +74,49 @@ __device__ __forceinline__ bool spline_frac( double x0, int num_node, int& base,
 
 // The 1-D B-spline weight of stencil node j as a polynomial of the
 // fractional offset f: w_j(f) = sum_n spline_coef(ORDER, j, n) t^n with
-// t = f - spline_shift(ORDER, j).  This is Spline<1>, <2> (dual grid) and
-// <3>::value of Cabana_Grid_Splines.hpp (:170-213, :272-324, :392-451)
-// re-expanded around a point where the weight's polynomial piece is exact
-// to evaluate: the edge nodes' weights, 0.5 (1 - f)^2 and 0.5 f^2 for the
-// quadratic spline, (1 - f)^3 / 6 and f^3 / 6 for the cubic one, become
-// a single power of t, so a weight near zero keeps its relative accuracy
-// (a sum of O(1) terms would leave an absolute error of one rounding of 1
-// in FP32, enough to change a nearly empty node's mass by O(1)).
+// t = f - spline_shift(ORDER, j).  These are the weights of Spline<0> ..
+// Spline<6>::value of Cabana_Grid_Splines.hpp (:94-100, :184-190, :281-299,
+// :394-428, :522-557, :668-708, :833-881), written there as polynomials in
+// x0 - int(x0) (minus 0.5 for orders 4 to 6) and here re-expanded around the
+// end of [0, 1) where the node's weight is smallest (the middle for the
+// centre node of an even order): the edge nodes' weights, (1 - f)^N / N! and
+// f^N / N!, become a single power of t, and a small weight keeps its
+// relative accuracy (a sum of O(1) terms would leave an absolute error of
+// one rounding of 1 in FP32, enough to change a nearly empty node's mass by
+// O(1)).
 __host__ __device__ constexpr double spline_shift( int order, int j )
 {
-    return order == 1   ? ( j == 0 ? 1.0 : 0.0 )
-           : order == 2 ? ( j == 0 ? 1.0 : ( j == 1 ? 0.5 : 0.0 ) )
-                        : ( ( j == 0 || j == 2 ) ? 1.0 : 0.0 );
+    return 2 * j < order ? 1.0 : ( 2 * j == order ? 0.5 : 0.0 );
 }
+
+// Coefficient of t^n in w_j.  Node j of a degree-N spline sees the
+// polynomial piece w_j(f) = 1/N! sum_{k=0}^{N-j} (-1)^k C(N+1, k)
+// (f + N - j - k)^N of the uniform B-spline (truncated-power form); with
+// f = t + s each power expands binomially.  Every partial sum is a dyadic
+// rational with fewer than 53 significant bits, so the sum is exact and
+// the one division by N! rounds correctly.  Evaluated at compile time only
+// (every caller binds it to a constexpr).
 __host__ __device__ constexpr double spline_coef( int order, int j, int n )
 {
-    if ( order == 1 )
-        return n == 1 ? ( j == 0 ? -1.0 : 1.0 ) : 0.0;
-    if ( order == 2 )
+    const double s = spline_shift( order, j );
+    double fact = 1.0;
+    for ( int i = 2; i <= order; ++i )
+        fact *= i;
+    double binom_n = 1.0; // C(order, n)
+    for ( int i = 0; i < n; ++i )
+        binom_n = binom_n * ( order - i ) / ( i + 1 );
+    double sum = 0.0;
+    double binom_k = 1.0; // C(order + 1, k)
+    for ( int k = 0; k <= order - j; ++k )
     {
-        if ( j == 1 )
-            return n == 0 ? 0.75 : ( n == 2 ? -1.0 : 0.0 );
-        return n == 2 ? 0.5 : 0.0;
+        const double c = s + ( order - j - k );
+        double p = 1.0;
+        for ( int e = 0; e < order - n; ++e )
+            p *= c;
+        sum += ( k % 2 ? -1.0 : 1.0 ) * binom_k * binom_n * p;
+        binom_k = binom_k * ( order + 1 - k ) / ( k + 1 );
     }
-    // Cubic: t^3 / 6 at the edges (with sign), 2/3 - t^2 +- t^3 / 2 inside.
-    if ( j == 0 || j == 3 )
-        return n == 3 ? ( j == 0 ? -1.0 / 6.0 : 1.0 / 6.0 ) : 0.0;
-    return n == 0 ? 2.0 / 3.0 : ( n == 2 ? -1.0 : ( n == 3 ? ( j == 1 ? 0.5 : -0.5 ) : 0.0 ) );
+    return sum / fact;
 }
 
 // Value, first and second derivative (in f) of node j's weight polynomial
@@ -108,33 +124,44 @@ __host__ __device__ constexpr double spline_coef( int order, int j, int n )
 template <int ORDER, int J, class Real>
 __device__ __forceinline__ void spline_node_poly( Real t, Real& w, Real& d1, Real& d2 )
 {
-    w = Real( spline_coef( ORDER, J, ORDER ) );
-#pragma unroll
-    for ( int n = ORDER - 1; n >= 0; --n )
-        w = w * t + Real( spline_coef( ORDER, J, n ) );
-    d1 = Real( ORDER * spline_coef( ORDER, J, ORDER ) );
-#pragma unroll
-    for ( int n = ORDER - 1; n >= 1; --n )
-        d1 = d1 * t + Real( n * spline_coef( ORDER, J, n ) );
+    constexpr double cN = spline_coef( ORDER, J, ORDER );
+    w = Real( cN );
+    static_for<ORDER>( [&]( auto I ) {
+        constexpr int n = ORDER - 1 - I;
+        constexpr double c = spline_coef( ORDER, J, n );
+        w = w * t + Real( c );
+    } );
+    d1 = Real( ORDER * cN );
+    if constexpr ( ORDER >= 2 )
+        static_for<ORDER - 1>( [&]( auto I ) {
+            constexpr int n = ORDER - 1 - I;
+            constexpr double c = n * spline_coef( ORDER, J, n );
+            d1 = d1 * t + Real( c );
+        } );
     d2 = Real( 0 );
     if constexpr ( ORDER >= 2 )
     {
-        d2 = Real( ORDER * ( ORDER - 1 ) * spline_coef( ORDER, J, ORDER ) );
-#pragma unroll
-        for ( int n = ORDER - 1; n >= 2; --n )
-            d2 = d2 * t + Real( n * ( n - 1 ) * spline_coef( ORDER, J, n ) );
+        d2 = Real( ORDER * ( ORDER - 1 ) * cN );
+        if constexpr ( ORDER >= 3 )
+            static_for<ORDER - 2>( [&]( auto I ) {
+                constexpr int n = ORDER - 1 - I;
+                constexpr double c = n * ( n - 1 ) * spline_coef( ORDER, J, n );
+                d2 = d2 * t + Real( c );
+            } );
     }
 }
 
-// Spline<ORDER>::value/gradient (Cabana_Grid_Splines.hpp <1> :170-213, <2>
-// :272-324, <3> :392-451) and second derivatives from the fractional offset.
+// Spline<ORDER>::value/gradient (Cabana_Grid_Splines.hpp <0> :94-117, <1>
+// :184-213, <2> :281-324, <3> :394-451, <4> :522-597, <5> :668-760, <6>
+// :833-938) and second derivatives from the fractional offset.
 template <int ORDER, class Real, bool NEED_H>
 __device__ __forceinline__ void spline_weights( Real f, Real rdx,
                                                 Spline1D<ORDER, Real>& s )
 {
     static_for<ORDER + 1>( [&]( auto J ) {
         constexpr int j = J;
-        const Real t = f - Real( spline_shift( ORDER, j ) );
+        constexpr double sh = spline_shift( ORDER, j );
+        const Real t = f - Real( sh );
         Real w, d1, d2;
         spline_node_poly<ORDER, j, Real>( t, w, d1, d2 );
         s.w[j] = w;
@@ -196,7 +223,7 @@ __device__ __forceinline__ bool stencil_base( const MeshDev& m,
         const double x0 = logical_coord<ORDER>( px[d], m.low[d], m.rdx[d] );
         const bool fin = x0 >= 0.0 && x0 < (double)m.num_node[d];
         const int i0 = fin ? (int)x0 : 0;
-        b[d] = ( ORDER >= 2 ) ? i0 - 1 : i0;
+        b[d] = i0 - ORDER / 2;
         ok = ok && fin && b[d] >= 0 && b[d] + K <= m.num_node[d];
     }
     return ok;

@@ -365,6 +365,11 @@ CABANA_AMD_DECLARE_TILED( o3 )
 CABANA_AMD_DECLARE_LAUNCHERS( o1 )
 CABANA_AMD_DECLARE_LAUNCHERS( o2 )
 CABANA_AMD_DECLARE_LAUNCHERS( o3 )
+// Orders 0 and 4 to 6 run the per-particle (untiled) kernels only.
+CABANA_AMD_DECLARE_LAUNCHERS( o0 )
+CABANA_AMD_DECLARE_LAUNCHERS( o4 )
+CABANA_AMD_DECLARE_LAUNCHERS( o5 )
+CABANA_AMD_DECLARE_LAUNCHERS( o6 )
 
 // Halo kernels on a ghosted node grid.
 hipError_t launch_halo_gather( Launcher& L, bool fp64, const MeshDev& m,

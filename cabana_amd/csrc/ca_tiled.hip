@@ -600,19 +600,18 @@ struct NodePoly
     Real g[ORDER];
     __device__ __forceinline__ void init( int j, Real rdx )
     {
-        shift = Real( spline_shift( ORDER, 0 ) );
-#pragma unroll
-        for ( int jj = 1; jj < K; ++jj )
-            shift = ( j == jj ) ? Real( spline_shift( ORDER, jj ) ) : shift;
-#pragma unroll
-        for ( int n = 0; n <= ORDER; ++n )
-        {
-            Real c = Real( spline_coef( ORDER, 0, n ) );
-#pragma unroll
-            for ( int jj = 1; jj < K; ++jj )
-                c = ( j == jj ) ? Real( spline_coef( ORDER, jj, n ) ) : c;
-            w[n] = c;
-        }
+        // Node j's shift and coefficients, selected from compile-time
+        // constants.
+        static_for<K>( [&]( auto JJ ) {
+            constexpr int jj = JJ;
+            constexpr double sh = spline_shift( ORDER, jj );
+            shift = ( jj == 0 || j == jj ) ? Real( sh ) : shift;
+            static_for<ORDER + 1>( [&]( auto N ) {
+                constexpr int n = N;
+                constexpr double c = spline_coef( ORDER, jj, n );
+                w[n] = ( jj == 0 || j == jj ) ? Real( c ) : w[n];
+            } );
+        } );
 #pragma unroll
         for ( int n = 1; n <= ORDER; ++n )
             g[n - 1] = Real( n ) * w[n] * rdx;

@@ -51,8 +51,8 @@ class Spline:
     """Spline<Order> tag (Cabana_Grid_Splines.hpp)."""
 
     def __init__(self, order):
-        if order not in (1, 2, 3):
-            raise ValueError("cabana_amd supports Spline<1>, Spline<2> and Spline<3>")
+        if order not in range(7):
+            raise ValueError("cabana_amd supports Spline<0> to Spline<6>")
         self.order = order
         self.num_knot = order + 1
 

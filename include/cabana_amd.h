@@ -8,7 +8,7 @@
  *
  * Reference interfaces (paths relative to the Cabana source tree):
  *   grid/src/Cabana_Grid_Interpolation.hpp   p2g / g2p drivers and functors
- *   grid/src/Cabana_Grid_Splines.hpp         Spline<1..3>, evaluateSpline
+ *   grid/src/Cabana_Grid_Splines.hpp         Spline<0..6>, evaluateSpline
  *   grid/src/Cabana_Grid_HaloBase.hpp,
  *   grid/src/impl/Cabana_Grid_Halo_Mpi.hpp   node halo gather / scatter(Sum)
  *   core/src/Cabana_LinkedCellList.hpp,
@@ -206,7 +206,9 @@ cabana_amd_status cabana_amd_check_domain( cabana_amd_context* ctx,
  * ghost nodes within halo_width are summed into their periodic owners
  * (Halo::scatter with ScatterReduce::Sum).  Ghost nodes keep their partial
  * sums.  halo_width < 0 means the full halo cell width.  The spline order is
- * 1, 2 or 3; points are FLOAT64 (num_comp 3).  Precision: FLOAT64 values and
+ * 0 to 6 (Spline<0> .. Spline<6>; orders 1 to 3 have the cell-tiled kernels
+ * for cell-sorted particles, orders 0 and 4 to 6 run per particle); points
+ * are FLOAT64 (num_comp 3).  Precision: FLOAT64 values and
  * grids give the plain FP64 path; FLOAT32 values and grids give the mixed
  * path (FP64 cell-local offsets, FP32 weights and accumulation). */
 cabana_amd_status cabana_amd_p2g( cabana_amd_context* ctx,

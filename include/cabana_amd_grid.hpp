@@ -52,8 +52,8 @@ namespace grid
 template <int Order>
 struct Spline
 {
-    static_assert( Order >= 1 && Order <= 3,
-                   "cabana_amd implements Spline<1>, Spline<2>, Spline<3>" );
+    static_assert( Order >= 0 && Order <= 6,
+                   "cabana_amd implements Spline<0> to Spline<6>" );
     static constexpr int order = Order;
 };
 

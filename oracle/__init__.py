@@ -25,6 +25,8 @@ G2P_OPS = {"ScalarValue": 0, "VectorValue": 1, "ScalarGradient": 2, "VectorGradi
 # (particle components, grid dofs)
 P2G_SHAPES = {0: (1, 1), 1: (3, 3), 2: (1, 3), 3: (3, 1), 4: (9, 3)}
 G2P_SHAPES = {0: (1, 1), 1: (3, 3), 2: (3, 1), 3: (9, 3), 4: (1, 3)}
+# Knots of the highest spline order (ORC_MAX_KNOT, Spline<6>).
+MAX_KNOT = 7
 
 
 class Mesh(ctypes.Structure):
@@ -45,9 +47,9 @@ class Grid(ctypes.Structure):
 
 class SplineData(ctypes.Structure):
     _fields_ = [("order", c_int), ("num_knot", c_int), ("dx", c_double * 3),
-                ("rdx", c_double * 3), ("x", c_double * 3), ("s", (c_int * 4) * 3),
-                ("w", (c_double * 4) * 3), ("g", (c_double * 4) * 3),
-                ("d", (c_double * 4) * 3)]
+                ("rdx", c_double * 3), ("x", c_double * 3), ("s", (c_int * MAX_KNOT) * 3),
+                ("w", (c_double * MAX_KNOT) * 3), ("g", (c_double * MAX_KNOT) * 3),
+                ("d", (c_double * MAX_KNOT) * 3)]
 
 
 class CartGrid(ctypes.Structure):

@@ -65,29 +65,123 @@ void orc_node_coordinates( const orc_mesh* m, const int idx[3], double x[3] )
 
 int orc_spline_num_knot( int order ) { return order + 1; }
 
+/* Spline<4>, <5>, <6> (Cabana_Grid_Splines.hpp:456-598, :600-761, :763-939)
+ * write every knot's value and gradient as a polynomial in
+ * xn = x0 - int(x0) - 0.5 with integer coefficients over a per-knot
+ * denominator.  These are those polynomials, lowest power first: value
+ * numerators and denominators (Spline<4>::value :522-557, Spline<5>::value
+ * :668-708, Spline<6>::value :833-881) and gradient ones, in units of rdx
+ * (Spline<4>::gradient :568-597, Spline<5>::gradient :720-760,
+ * Spline<6>::gradient :893-938). */
+static const double hi_val4[5][5] = { { 1, -8, 24, -32, 16 },
+                                      { 19, -44, 24, 16, -16 },
+                                      { 115, 0, -120, 0, 48 },
+                                      { 19, 44, 24, -16, -16 },
+                                      { 1, 8, 24, 32, 16 } };
+static const double hi_vden4[5] = { 384, 96, 192, 96, 384 };
+static const double hi_grad4[5][4] = { { -1, 6, -12, 8 },
+                                       { -11, 12, 12, -16 },
+                                       { 0, -5, 0, 4 },
+                                       { 11, 12, -12, -16 },
+                                       { 1, 6, 12, 8 } };
+static const double hi_gden4[5] = { 48, 24, 4, 24, 48 };
+
+static const double hi_val5[6][6] = { { 1, -10, 40, -80, 80, -32 },
+                                      { 237, -750, 840, -240, -240, 160 },
+                                      { 841, -770, -440, 560, 80, -160 },
+                                      { 841, 770, -440, -560, 80, 160 },
+                                      { 237, 750, 840, 240, -240, -160 },
+                                      { 1, 10, 40, 80, 80, 32 } };
+static const double hi_vden5[6] = { 3840, 3840, 1920, 1920, 3840, 3840 };
+static const double hi_grad5[6][5] = { { -1, 8, -24, 32, -16 },
+                                       { -75, 168, -72, -96, 80 },
+                                       { -77, -88, 168, 32, -80 },
+                                       { 77, -88, -168, 32, 80 },
+                                       { 75, 168, 72, -96, -80 },
+                                       { 1, 8, 24, 32, 16 } };
+static const double hi_gden5[6] = { 384, 384, 192, 192, 384, 384 };
+
+static const double hi_val6[7][7] = {
+    { 1, -12, 60, -160, 240, -192, 64 },
+    { 361, -1416, 2220, -1600, 240, 384, -192 },
+    { 10543, -17340, 4740, 6880, -4080, -960, 960 },
+    { 5887, 0, -4620, 0, 1680, 0, -320 },
+    { 10543, 17340, 4740, -6880, -4080, 960, 960 },
+    { 361, 1416, 2220, 1600, 240, -384, -192 },
+    { 1, 12, 60, 160, 240, 192, 64 } };
+static const double hi_vden6[7] = { 46080, 23040, 46080, 11520, 46080, 23040, 46080 };
+static const double hi_grad6[7][6] = { { -1, 10, -40, 80, -80, 32 },
+                                       { -59, 185, -200, 40, 80, -48 },
+                                       { -289, 158, 344, -272, -80, 96 },
+                                       { 0, -77, 0, 56, 0, -16 },
+                                       { 289, 158, -344, -272, 80, 96 },
+                                       { 59, 185, 200, 40, -80, -48 },
+                                       { 1, 10, 40, 80, 80, 32 } };
+static const double hi_gden6[7] = { 3840, 960, 768, 96, 768, 960, 3840 };
+
+/* Numerator table row j (num_coef coefficients, lowest power first) of the
+ * value (deriv 0), gradient (1) or second derivative (2, the derivative of
+ * the gradient polynomial) of a degree 4-6 spline, evaluated at xn by
+ * Horner's rule and divided by the knot's denominator. */
+static double hi_poly( int order, int deriv, int j, double xn )
+{
+    const double* c;
+    double den;
+    int nc;
+    int d = deriv;
+    if ( d == 0 )
+    {
+        nc = order + 1;
+        c = order == 4 ? hi_val4[j] : order == 5 ? hi_val5[j] : hi_val6[j];
+        den = order == 4 ? hi_vden4[j] : order == 5 ? hi_vden5[j] : hi_vden6[j];
+    }
+    else
+    {
+        nc = order;
+        c = order == 4 ? hi_grad4[j] : order == 5 ? hi_grad5[j] : hi_grad6[j];
+        den = order == 4 ? hi_gden4[j] : order == 5 ? hi_gden5[j] : hi_gden6[j];
+        d -= 1;
+    }
+    /* d = 0: the polynomial itself; d = 1: its derivative in xn. */
+    double acc = 0.0;
+    for ( int n = nc - 1; n >= d; --n )
+        acc = acc * xn + ( d ? n * c[n] : c[n] );
+    return acc / den;
+}
+
 /* mapToLogicalGrid: dual grid (+0.5) for even orders, primal for odd.
- * Spline<0> :59-63, Spline<1> :141-145, Spline<2> :239-244, Spline<3> :349-353 */
+ * Spline<0> :59-63, Spline<1> :141-145, Spline<2> :239-244, Spline<3> :349-353,
+ * Spline<4> :477-481, Spline<5> :621-625, Spline<6> :784-788 */
 double orc_spline_map( int order, double xp, double rdx, double low_x )
 {
-    if ( order == 0 || order == 2 )
+    if ( order % 2 == 0 )
         return ( xp - low_x ) * rdx + 0.5;
     return ( xp - low_x ) * rdx;
 }
 
-/* stencil: Spline<0> :79-84, <1> :163-169, <2> :264-271, <3> :383-391 */
+/* stencil: Spline<0> :79-84, <1> :163-169, <2> :264-271, <3> :383-391,
+ * <4> :504-512 (int(x0) - 2), <5> :649-659 (int(x0) - 2), <6> :813-824
+ * (int(x0) - 3) */
 void orc_spline_stencil( int order, double x0, int* idx )
 {
     int base = (int)x0;
     if ( order >= 2 )
-        base -= 1;
+        base -= order / 2;
     for ( int n = 0; n <= order; ++n )
         idx[n] = base + n;
 }
 
-/* value: Spline<0> :95-100, <1> :178-187, <2> :280-298, <3> :400-427 */
+/* value: Spline<0> :95-100, <1> :178-187, <2> :280-298, <3> :400-427,
+ * <4>..<6> in hi_poly */
 void orc_spline_value( int order, double x0, double* w )
 {
-    if ( order == 0 )
+    if ( order >= 4 )
+    {
+        const double xn = x0 - (int)x0 - 0.5;
+        for ( int j = 0; j <= order; ++j )
+            w[j] = hi_poly( order, 0, j, xn );
+    }
+    else if ( order == 0 )
     {
         w[0] = 1.0;
     }
@@ -127,10 +221,17 @@ void orc_spline_value( int order, double x0, double* w )
     }
 }
 
-/* gradient: Spline<0> :111-117, <1> :199-213, <2> :309-324, <3> :437-451 */
+/* gradient: Spline<0> :111-117, <1> :199-213, <2> :309-324, <3> :437-451,
+ * <4>..<6> in hi_poly */
 void orc_spline_gradient( int order, double x0, double rdx, double* g )
 {
-    if ( order == 0 )
+    if ( order >= 4 )
+    {
+        const double xn = x0 - (int)x0 - 0.5;
+        for ( int j = 0; j <= order; ++j )
+            g[j] = hi_poly( order, 1, j, xn ) * rdx;
+    }
+    else if ( order == 0 )
     {
         g[0] = 0.0;
     }
@@ -165,7 +266,13 @@ void orc_spline_gradient( int order, double x0, double rdx, double* g )
 void orc_spline_hessian( int order, double x0, double rdx, double* h )
 {
     double r2 = rdx * rdx;
-    if ( order <= 1 )
+    if ( order >= 4 )
+    {
+        const double xn = x0 - (int)x0 - 0.5;
+        for ( int j = 0; j <= order; ++j )
+            h[j] = hi_poly( order, 2, j, xn ) * r2;
+    }
+    else if ( order <= 1 )
     {
         for ( int n = 0; n <= order; ++n )
             h[n] = 0.0;
@@ -640,7 +747,7 @@ static int check_p2g( const orc_mesh* m, int order, int op,
                       const orc_pfield* points, const orc_pfield* values,
                       const orc_grid* grid )
 {
-    if ( order < 0 || order > 3 || op < 0 || op > 4 )
+    if ( order < 0 || order > 6 || op < 0 || op > 4 )
         return 1;
     if ( points->ncomp != 3 || values->ncomp != p2g_point_comps( op ) ||
          values->n < points->n )
@@ -652,7 +759,7 @@ static int check_g2p( const orc_mesh* m, int order, int op,
                       const orc_pfield* points, const orc_pfield* values,
                       const orc_grid* grid )
 {
-    if ( order < 0 || order > 3 || op < 0 || op > 4 )
+    if ( order < 0 || order > 6 || op < 0 || op > 4 )
         return 1;
     if ( points->ncomp != 3 || values->ncomp != g2p_point_comps( op ) ||
          values->n < points->n )
@@ -801,7 +908,7 @@ int orc_g2p_omp( const orc_mesh* m, int order, int op, double multiplier,
 
 /* Weight W = wi wj wk of one stencil node, its position gradient dW[e] and
  * Hessian ddW[e][f] (all in the physical frame). */
-static void node_derivs( const orc_spline_data* sd, const double h[3][4],
+static void node_derivs( const orc_spline_data* sd, const double h[3][ORC_MAX_KNOT],
                          int i, int j, int k, double* W, double dW[3],
                          double ddW[3][3] )
 {
@@ -900,7 +1007,7 @@ int orc_p2g_adjoint( const orc_mesh* m, int order, int op, double multiplier,
         load_point( points, p, px );
         orc_spline_data sd;
         orc_evaluate_spline( m, order, px, &sd );
-        double hs[3][4];
+        double hs[3][ORC_MAX_KNOT];
         for ( int d = 0; d < 3; ++d )
             orc_spline_hessian( order, sd.x[d], sd.rdx[d], hs[d] );
         double v[9];
@@ -1024,7 +1131,7 @@ int orc_g2p_adjoint( const orc_mesh* m, int order, int op, double multiplier,
         load_point( points, p, px );
         orc_spline_data sd;
         orc_evaluate_spline( m, order, px, &sd );
-        double hs[3][4];
+        double hs[3][ORC_MAX_KNOT];
         for ( int d = 0; d < 3; ++d )
             orc_spline_hessian( order, sd.x[d], sd.rdx[d], hs[d] );
         double u[9];

@@ -57,6 +57,9 @@ typedef struct
 } orc_grid;
 
 /* Per-point spline data, grid/src/Cabana_Grid_Splines.hpp:1051-1134. */
+/* Knots of the highest spline order (Spline<6>). */
+#define ORC_MAX_KNOT 7
+
 typedef struct
 {
     int order;
@@ -64,10 +67,10 @@ typedef struct
     double dx[3];
     double rdx[3];
     double x[3];
-    int s[3][4];
-    double w[3][4];
-    double g[3][4];
-    double d[3][4];
+    int s[3][ORC_MAX_KNOT];
+    double w[3][ORC_MAX_KNOT];
+    double g[3][ORC_MAX_KNOT];
+    double d[3][ORC_MAX_KNOT];
 } orc_spline_data;
 
 /* Operator ids: the ten functors of grid/src/Cabana_Grid_Interpolation.hpp. */

@@ -21,10 +21,14 @@ def bspline(p, t):
     for k in range(p + 2):
         u = t + (p + 1) / 2.0 - k
         acc += (-1) ** k * math.comb(p + 1, k) * np.where(u > 0, u, 0.0) ** p
-    return acc / math.factorial(p)
+    # B_p vanishes for |t| >= (p + 1) / 2; there the alternating sum of large
+    # powers would only leave its rounding error.
+    return np.where(np.abs(t) < (p + 1) / 2.0, acc / math.factorial(p), 0.0)
 
 
 def dbspline(p, t):
+    if p == 0:
+        return np.zeros_like(np.asarray(t, dtype=np.float64))
     return bspline(p - 1, np.asarray(t) + 0.5) - bspline(p - 1, np.asarray(t) - 0.5)
 
 

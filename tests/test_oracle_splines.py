@@ -15,9 +15,11 @@ def grid_func(x):
     return 4.32 * x - 0.31
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", range(7))
 def test_partition_of_unity(oracle, order):
-    # tstSplines.cpp:86-116 (linear), :148-178 (quadratic), :216-246 (cubic)
+    # tstSplines.cpp:27-55 (zero), :86-116 (linear), :148-178 (quadratic),
+    # :216-246 (cubic), :286-314 (quartic), :361-389 (quintic), :439-467
+    # (sextic)
     for xp in (-1.4, 2.1789, LOW_X + 5 * DX):
         _, _, w, _, _ = oracle.spline_1d(order, xp, RDX, LOW_X)
         assert sum(w) == pytest.approx(1.0, rel=1e-14)
@@ -60,24 +62,57 @@ def test_cubic_stencil_and_reproduction(oracle):
     assert np.dot(field, g) == pytest.approx(4.32, rel=1e-12)
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+def test_zero_stencil_and_reproduction(oracle):
+    # tstSplines.cpp:57-81: point at node 4 + 0.25 dx; the one knot takes the
+    # node's value and the gradient vanishes.
+    node_id = 4
+    xp = LOW_X + (node_id + 0.25) * DX
+    x0, s, w, g, h = oracle.spline_1d(0, xp, RDX, LOW_X)
+    assert int(x0) == node_id and s == [node_id]
+    field = grid_func(LOW_X + node_id * DX)
+    assert field * w[0] == field
+    assert field * g[0] == 0.0 and h == [0.0]
+
+
+@pytest.mark.parametrize("order,offset,first", [(4, 0.25, -2), (5, 0.75, -2), (6, 0.25, -3)])
+def test_high_order_stencil_and_reproduction(oracle, order, offset, first):
+    # tstSplines.cpp:316-356 (quartic: node 4 + 0.25 dx, nodes 2..6),
+    # :391-434 (quintic: cell 4 + 0.75 dx, nodes 2..7), :469-517 (sextic:
+    # node 4 + 0.25 dx, nodes 1..7); linear functions are reproduced.
+    cid = 4
+    xp = LOW_X + (cid + offset) * DX
+    x0, s, w, g, _ = oracle.spline_1d(order, xp, RDX, LOW_X)
+    nodes = [cid + first + n for n in range(order + 1)]
+    assert [int(x0) + first + n for n in range(order + 1)] == nodes
+    assert s == nodes
+    field = [grid_func(LOW_X + n * DX) for n in nodes]
+    assert np.dot(field, w) == pytest.approx(grid_func(xp), rel=1e-13)
+    assert np.dot(field, g) == pytest.approx(4.32, rel=1e-12)
+
+
+@pytest.mark.parametrize("order", range(7))
 def test_weights_match_textbook_bsplines(oracle, order):
     """Weights, gradients and second derivatives equal B_p(x - n), B_p' rdx
-    and B_p'' rdx^2 at the stencil nodes (truncated-power formula)."""
+    and B_p'' rdx^2 at the stencil nodes (truncated-power formula, whose
+    alternating sum loses about 1e-13 to cancellation at degree 6)."""
     rng = np.random.default_rng(7 + order)
+    atol = 1e-13 if order <= 3 else 1e-12
     for xp in rng.uniform(LOW_X + 3 * DX, LOW_X + 40 * DX, size=200):
         _, s, w, g, h = oracle.spline_1d(order, xp, RDX, LOW_X)
         xnode = (xp - LOW_X) * RDX
         t = xnode - np.array(s, dtype=np.float64)
-        np.testing.assert_allclose(w, bspline(order, t), rtol=0, atol=1e-13)
+        np.testing.assert_allclose(w, bspline(order, t), rtol=0, atol=atol)
+        if order == 0:
+            assert g == [0.0] and h == [0.0]
+            continue
         np.testing.assert_allclose(g, dbspline(order, t) * RDX, rtol=0, atol=1e-11)
         if order >= 2:
             np.testing.assert_allclose(h, ddbspline(order, t) * RDX ** 2, rtol=0, atol=1e-9)
         else:
-            assert h == [0.0, 0.0]
+            assert h == [0.0] * (order + 1)
 
 
-@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("order", [2, 3, 4, 5, 6])
 def test_hessian_is_derivative_of_gradient(oracle, order):
     """Second derivatives (not in the reference) against central differences
     of Spline<N>::gradient."""
