@@ -10,6 +10,6 @@ from .grid import (  # noqa: F401
     createScalarValueG2P, createScalarValueP2G, createTensorDivergenceP2G,
     createUniformGlobalMesh, createVectorDivergenceG2P, createVectorDivergenceP2G,
     createVectorGradientG2P, createVectorValueG2P, createVectorValueP2G, g2p, g2p_adjoint,
-    grid_descriptor, p2g, p2g_adjoint, particle_descriptor)
+    grid_descriptor, p2g, p2g_adjoint, particleMigrate, particle_descriptor)
 from .linkedcell import LinkedCellList, createLinkedCellList, permute  # noqa: F401
 from . import autograd  # noqa: F401,E402

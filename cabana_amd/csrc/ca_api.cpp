@@ -1833,6 +1833,133 @@ extern "C" cabana_amd_status cabana_amd_permute_members(
 }
 
 // ---------------------------------------------------------------------------
+// Particle migration (one rank)
+// ---------------------------------------------------------------------------
+
+namespace
+{
+// Reads one device counter back (the call waits for the stream, as the
+// reference's migrateCount waits for its reduction).
+cabana_amd_status read_counter( cabana_amd_context* ctx, hipStream_t s,
+                                unsigned long long& v )
+{
+    CA_HIP( ctx, hipMemcpyAsync( &v, ctx->bad, sizeof( v ), hipMemcpyDeviceToHost, s ),
+            "hipMemcpyAsync" );
+    CA_HIP( ctx, hipStreamSynchronize( s ), "hipStreamSynchronize" );
+    return CABANA_AMD_OK;
+}
+} // namespace
+
+extern "C" cabana_amd_status cabana_amd_particle_migrate(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+    const cabana_amd_particle_field* positions, const cabana_amd_particle_field* members,
+    int32_t num_members, int32_t min_halo_width, int32_t force_migrate,
+    int64_t* num_particle, int32_t* migrated, cabana_amd_stream stream )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    if ( !positions || !num_particle || !migrated || num_members < 0 ||
+         ( num_members > 0 && !members ) )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "NULL argument" );
+    if ( min_halo_width < 0 )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "negative minimum halo width" );
+    MeshDev m;
+    CA_CHECK( make_mesh( ctx, mesh, m ) );
+    const int64_t n = positions->num_particle;
+    if ( n <= 0 )
+    {
+        // No particle is outside the band; a forced migration moves none.
+        *num_particle = 0;
+        *migrated = force_migrate ? 1 : 0;
+        return CABANA_AMD_OK;
+    }
+    PFieldDev pos;
+    CA_CHECK( make_pfield( ctx, *positions, 3, CABANA_AMD_FLOAT64, n, false, "positions",
+                           pos ) );
+    std::vector<PFieldDev> f( (size_t)num_members );
+    std::vector<int> es( (size_t)num_members );
+    for ( int k = 0; k < num_members; ++k )
+    {
+        if ( members[k].num_particle < n )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "a member holds fewer particles than the positions" );
+        es[k] = dtype_size( members[k].dtype );
+        if ( !es[k] )
+            return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED, "unknown dtype" );
+        CA_CHECK( make_pfield( ctx, members[k], -1, -1, n, false, "member", f[k] ) );
+    }
+    // LocalMesh (Cabana_Grid_LocalMesh.hpp:61-107) and GlobalMesh corners of
+    // one rank, in the same FP64 operations.
+    MigrateBoundsDev b;
+    for ( int d = 0; d < 3; ++d )
+    {
+        const double cell = mesh->cell_size[d];
+        const double own_lo = mesh->low_corner[d] + cell * 0;
+        const double own_hi = mesh->low_corner[d] + cell * ( 0 + m.num_cell[d] );
+        const double ghost_lo = own_lo - mesh->halo_cell_width * cell;
+        const double ghost_hi = own_hi + mesh->halo_cell_width * cell;
+        b.band_lo[d] = ghost_lo + min_halo_width * cell;
+        b.band_hi[d] = ghost_hi - min_halo_width * cell;
+        b.own_lo[d] = own_lo;
+        b.own_hi[d] = own_hi;
+        b.global_lo[d] = mesh->low_corner[d];
+        b.global_hi[d] = mesh->high_corner[d];
+        b.extent[d] = mesh->high_corner[d] - mesh->low_corner[d];
+        b.periodic[d] = m.periodic[d];
+    }
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    *num_particle = n;
+    *migrated = 0;
+    if ( !force_migrate )
+    {
+        CA_HIP( ctx, launch_fill_i32( *ctx, reinterpret_cast<int32_t*>( ctx->bad ), 2, 0, s ),
+                "fill" );
+        CA_HIP( ctx, launch_migrate_count( *ctx, pos, n, b, ctx->bad, s ), "migrate count" );
+        unsigned long long outside = 0;
+        CA_CHECK( read_counter( ctx, s, outside ) );
+        if ( outside == 0 )
+            return CABANA_AMD_OK;
+    }
+    *migrated = 1;
+    const int64_t nblock = ( n + 1023 ) / 1024 + 1;
+    CA_HIP( ctx, ctx->ensure( SLOT_CELL_OF, sizeof( int32_t ) * ( n + 64 ) ), "scratch" );
+    CA_HIP( ctx, ctx->ensure( SLOT_CURSOR, sizeof( int64_t ) * 2 * ( n + 64 ) ), "scratch" );
+    CA_HIP( ctx, ctx->ensure( SLOT_SCAN, sizeof( int64_t ) * nblock ), "scratch" );
+    int32_t* keep = static_cast<int32_t*>( ctx->scratch[SLOT_CELL_OF] );
+    int64_t* rank = static_cast<int64_t*>( ctx->scratch[SLOT_CURSOR] );
+    int64_t* idx = rank + n + 64;
+    int64_t* sums = static_cast<int64_t*>( ctx->scratch[SLOT_SCAN] );
+    CA_HIP( ctx, launch_fill_i32( *ctx, reinterpret_cast<int32_t*>( ctx->bad ), 2, 0, s ),
+            "fill" );
+    CA_HIP( ctx, launch_migrate_shift( *ctx, pos, n, b, keep, ctx->bad, s ), "migrate shift" );
+    unsigned long long dropped = 0;
+    CA_CHECK( read_counter( ctx, s, dropped ) );
+    if ( dropped == 0 || num_members == 0 )
+    {
+        *num_particle = n - (int64_t)dropped;
+        return CABANA_AMD_OK;
+    }
+    // Distributor::migrate keeps the staying particles in their order: a
+    // stable compaction of every member.
+    const int64_t kept = n - (int64_t)dropped;
+    CA_HIP( ctx, launch_exclusive_scan( *ctx, keep, rank, n, sums, nblock, s ), "scan" );
+    CA_HIP( ctx, launch_compact_index( *ctx, keep, rank, n, idx, s ), "compact index" );
+    if ( kept > 0 )
+    {
+        size_t row_bytes = 0;
+        for ( int k = 0; k < num_members; ++k )
+            row_bytes += (size_t)es[k] * f[k].ncomp;
+        CA_HIP( ctx, ctx->ensure( SLOT_PERMUTE, row_bytes * (size_t)( kept + 1 ) ), "scratch" );
+        CA_HIP( ctx,
+                launch_permute_members( *ctx, f.data(), es.data(), num_members, idx, 0, kept,
+                                        ctx->scratch[SLOT_PERMUTE], s ),
+                "permute members" );
+    }
+    *num_particle = kept;
+    return CABANA_AMD_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Profiling
 // ---------------------------------------------------------------------------
 

@@ -684,6 +684,87 @@ __global__ __launch_bounds__( BLOCK ) void fill_i32_kernel( int32_t* dst,
         dst[i] = value;
 }
 
+
+// ---------------------------------------------------------------------------
+// Particle migration on one rank (Cabana_Grid_ParticleDistributor.hpp)
+// ---------------------------------------------------------------------------
+
+// Reports a wave's flagged lanes to one 64-bit counter with one atomic.
+__device__ __forceinline__ void count_flagged( bool flag, unsigned long long* counter )
+{
+    const unsigned long long m = __ballot( flag );
+    if ( m && (int)( threadIdx.x & 63 ) == __ffsll( (long long)m ) - 1 )
+        atomicAdd( counter, (unsigned long long)__popcll( m ) );
+}
+
+// migrateCount (:170-221): particles with a coordinate below the band's low
+// or above its high corner (a NaN coordinate compares false, as there).
+__global__ __launch_bounds__( BLOCK ) void migrate_count_kernel( PFieldDev pos, int64_t n,
+                                                                 MigrateBoundsDev b,
+                                                                 unsigned long long* count )
+{
+    const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( p >= n )
+        return;
+    const uint32_t a = pos.base( p );
+    bool out = false;
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        const double x = *pos.ptr<const double>( a, d );
+        out = out || x < b.band_lo[d] || x > b.band_hi[d];
+    }
+    count_flagged( out, count );
+}
+
+// getMigrateDestinations (:87-148) on one rank.  Every neighbour of a
+// periodic dimension is this rank; past the owned domain of a non-periodic
+// dimension there is no neighbour (rank -1), and Distributor removes the
+// particle.  The destination is taken from the unshifted position; then each
+// periodic coordinate above the global high corner moves down by the extent
+// and one below the low corner moves up (one shift, as there).
+__global__ __launch_bounds__( BLOCK ) void migrate_shift_kernel( PFieldDev pos, int64_t n,
+                                                                 MigrateBoundsDev b,
+                                                                 int32_t* keep,
+                                                                 unsigned long long* dropped )
+{
+    const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( p >= n )
+        return;
+    const uint32_t a = pos.base( p );
+    bool drop = false;
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        double* xp = pos.ptr<double>( a, d );
+        const double x = *xp;
+        const bool outside = x < b.own_lo[d] || x > b.own_hi[d];
+        if ( b.periodic[d] )
+        {
+            if ( x > b.global_hi[d] )
+                *xp = x - b.extent[d];
+            else if ( x < b.global_lo[d] )
+                *xp = x + b.extent[d];
+        }
+        else
+            drop = drop || outside;
+    }
+    keep[p] = drop ? 0 : 1;
+    count_flagged( drop, dropped );
+}
+
+// Ascending list of the kept particles: idx[rank[p]] = p for every kept p,
+// rank being the exclusive scan of the keep flags (the Serial order of the
+// particles that stay, Cabana_Distributor.hpp:80-101).
+__global__ __launch_bounds__( BLOCK ) void compact_index_kernel( const int32_t* keep,
+                                                                 const int64_t* rank, int64_t n,
+                                                                 int64_t* idx )
+{
+    const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( p < n && keep[p] )
+        idx[rank[p]] = p;
+}
+
 } // namespace
 
 hipError_t launch_halo_gather( Launcher& L, bool fp64, const MeshDev& m,
@@ -1101,6 +1182,44 @@ hipError_t launch_brick_parts( Launcher& L, const TileArgs& t,
     hipLaunchKernelGGL( part_fill_kernel, blocks_for( nbricks ), dim3( BLOCK ), 0, s,
                         brick_nparts, brick_start, (int)nbricks, parts, num_parts );
     L.after( "brick_parts", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_migrate_count( Launcher& L, const PFieldDev& pos, int64_t n,
+                                 const MigrateBoundsDev& b, unsigned long long* count,
+                                 hipStream_t s )
+{
+    if ( n <= 0 )
+        return hipSuccess;
+    L.before( "migrate_count", s );
+    hipLaunchKernelGGL( migrate_count_kernel, blocks_for( n ), dim3( BLOCK ), 0, s, pos, n, b,
+                        count );
+    L.after( "migrate_count", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_migrate_shift( Launcher& L, const PFieldDev& pos, int64_t n,
+                                 const MigrateBoundsDev& b, int32_t* keep,
+                                 unsigned long long* dropped, hipStream_t s )
+{
+    if ( n <= 0 )
+        return hipSuccess;
+    L.before( "migrate_shift", s );
+    hipLaunchKernelGGL( migrate_shift_kernel, blocks_for( n ), dim3( BLOCK ), 0, s, pos, n, b,
+                        keep, dropped );
+    L.after( "migrate_shift", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_index( Launcher& L, const int32_t* keep, const int64_t* rank,
+                                 int64_t n, int64_t* idx, hipStream_t s )
+{
+    if ( n <= 0 )
+        return hipSuccess;
+    L.before( "compact_index", s );
+    hipLaunchKernelGGL( compact_index_kernel, blocks_for( n ), dim3( BLOCK ), 0, s, keep, rank, n,
+                        idx );
+    L.after( "compact_index", s );
     return hipGetLastError();
 }
 

@@ -447,6 +447,34 @@ hipError_t launch_permute_copy_back( Launcher& L, const PFieldDev& f,
                                      hipStream_t s );
 hipError_t launch_fill_i32( Launcher& L, int32_t* dst, int64_t n,
                             int32_t value, hipStream_t s );
+
+// Bounds of particleMigrate on one rank (Cabana_Grid_ParticleDistributor.hpp
+// :87-148, :170-221), formed on the host as LocalMesh and GlobalMesh form
+// them: the band of migrateCount (ghosted corners moved in by
+// min_halo_width cells), the owned corners, the global corners and extent.
+struct MigrateBoundsDev
+{
+    double band_lo[3];
+    double band_hi[3];
+    double own_lo[3];
+    double own_hi[3];
+    double global_lo[3];
+    double global_hi[3];
+    double extent[3];
+    int32_t periodic[3];
+};
+// *count += particles outside the band.
+hipError_t launch_migrate_count( Launcher& L, const PFieldDev& pos, int64_t n,
+                                 const MigrateBoundsDev& b, unsigned long long* count,
+                                 hipStream_t s );
+// Periodic shift of the positions in place, keep[p] = 0 for particles
+// leaving through a non-periodic boundary, *dropped += their number.
+hipError_t launch_migrate_shift( Launcher& L, const PFieldDev& pos, int64_t n,
+                                 const MigrateBoundsDev& b, int32_t* keep,
+                                 unsigned long long* dropped, hipStream_t s );
+// idx[rank[p]] = p for the kept particles (rank: exclusive scan of keep).
+hipError_t launch_compact_index( Launcher& L, const int32_t* keep, const int64_t* rank,
+                                 int64_t n, int64_t* idx, hipStream_t s );
 // Part table of a brick tiling: brick_nparts (nbricks ints), brick_start
 // (nbricks int64), block_sums (scan scratch, nblock_cap int64), parts
 // (nbricks + n / cap entries at most) and num_parts (1 int).

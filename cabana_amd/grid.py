@@ -702,3 +702,38 @@ def check_domain():
     n = ctypes.c_int64(0)
     N.check(N.load().cabana_amd_check_domain(ctx, N.current_stream(), ctypes.byref(n)), ctx)
     return n.value
+
+
+# ---------------------------------------------------------------------------
+# Particle migration (one rank)
+# ---------------------------------------------------------------------------
+
+
+def particleMigrate(local_grid, positions, particles, min_halo_width, force_migrate=False):
+    """particleMigrate(local_grid, positions, particles, min_halo_width,
+    force_migrate) (Cabana_Grid_ParticleDistributor.hpp:330-352) on one rank.
+
+    Unless force_migrate is set, nothing changes while every particle lies
+    within the ghosted domain moved in by min_halo_width cells.  Otherwise
+    periodic coordinates are shifted back into the global box (in place in
+    ``positions``) and particles beyond a non-periodic boundary are removed
+    from every member of ``particles`` (a sequence of particle fields,
+    compacted in place in their order; include ``positions`` when it belongs
+    to the particle set).  Returns ``(migrated, num_particle)``: whether the
+    migration ran (the reference's return value) and how many particles
+    remain, which are the first ``num_particle`` rows of every member (a
+    torch tensor cannot shrink in place as an AoSoA resizes)."""
+    pos = particle_descriptor(positions, 3)
+    descs = [particle_descriptor(f) for f in particles]
+    if len({d.data for d in descs}) != len(descs):
+        raise ValueError("particleMigrate: the same field was passed twice")
+    arr = (type(descs[0]) * len(descs))(*descs) if descs else None
+    m = local_grid.mesh_descriptor()
+    ctx = N.context()
+    num = ctypes.c_int64(0)
+    migrated = ctypes.c_int32(0)
+    N.check(N.load().cabana_amd_particle_migrate(
+        ctx, ctypes.byref(m), ctypes.byref(pos), arr, len(descs), int(min_halo_width),
+        1 if force_migrate else 0, ctypes.byref(num), ctypes.byref(migrated),
+        N.current_stream()), ctx)
+    return bool(migrated.value), int(num.value)

@@ -10,7 +10,8 @@ One step, on a periodic uniform mesh with the quadratic spline:
      (u = 0), with no threshold;
   4. fused G2P of velocity (VectorValueG2P) and velocity gradient
      (VectorGradientG2P) from u;
-  5. advection x += dt * v with periodic wrap, v = G2P velocity.
+  5. advection x += dt * v, v = G2P velocity, and the periodic wrap of
+     particleMigrate (the library's one-rank migration).
 The backward pass goes through the hand-written adjoint kernels; with
 `checkpoint_every` > 0, only every k-th state is kept and the steps in
 between (including their spline weights) are recomputed in backward.
@@ -61,6 +62,18 @@ def sort_by_cell(box, x, *fields):
     return (lcl, permute_tensor(x, perm)) + tuple(permute_tensor(f, perm) for f in fields)
 
 
+def periodic_wrap(box, x):
+    """Positions moved back into the periodic box by particleMigrate
+    (Cabana_Grid_ParticleDistributor.hpp:330-352; every dimension is periodic,
+    so no particle is removed): a coordinate past either corner moves by one
+    extent.  The shift is constant per particle, so the gradient passes
+    through unchanged."""
+    y = x.detach().clone()
+    G.particleMigrate(box.local_grid, y, [], box.local_grid.haloCellWidth(),
+                      force_migrate=True)
+    return y + (x - x.detach())
+
+
 def explicit_step(box, x, v, m, dt=0.1, body_force=(0.0, 0.0, -1e-3), resort=True,
                   record=None, mass_floor=1e-3):
     """One transfer step; returns (x, v, m, velocity_gradient).  When
@@ -91,7 +104,7 @@ def explicit_step(box, x, v, m, dt=0.1, body_force=(0.0, 0.0, -1e-3), resort=Tru
     f = torch.tensor(body_force, dtype=mom.dtype, device=mom.device)
     u = (mom + (dt * f) * mass) / (mass + mass_floor)
     vn, grad_v = AG.g2p(x, [u, u], lg, ["VectorValue", "VectorGradient"], order=2, bins=bins)
-    xn = torch.remainder(x + dt * vn.to(x.dtype), box.length)
+    xn = periodic_wrap(box, x + dt * vn.to(x.dtype))
     return xn, vn, m, grad_v
 
 

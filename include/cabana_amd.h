@@ -370,6 +370,33 @@ cabana_amd_status cabana_amd_permute_members(
     cabana_amd_stream stream );
 
 /* ----------------------------------------------------------------------- */
+/* Particle migration (one rank)                                             */
+/* ----------------------------------------------------------------------- */
+
+/* particleMigrate(local_grid, positions, particles, min_halo_width,
+ * force_migrate) (grid/src/Cabana_Grid_ParticleDistributor.hpp:330-352)
+ * on one rank.  Unless force_migrate is set, nothing happens when every
+ * particle lies within the band migrateCount checks (:170-221: the ghosted
+ * domain moved in by min_halo_width cells); *migrated then stays 0.
+ * Otherwise (*migrated = 1) getMigrateDestinations (:87-148) runs: every
+ * periodic coordinate above the global high corner moves down by the
+ * extent, one below the low corner moves up (one shift), and particles
+ * beyond the domain in a non-periodic dimension, whose destination rank is
+ * -1, are removed by compacting every member in place, the remaining
+ * particles keeping their order (Distributor::migrate, Serial order,
+ * core/src/Cabana_Distributor.hpp:80-101).  positions (FLOAT64, num_comp 3)
+ * are shifted in place; list them among the members too when they belong
+ * to the particle set.  *num_particle returns the number of particles
+ * after the call.  Every member holds at least positions->num_particle
+ * particles; members must not share memory.  Waits for the stream. */
+cabana_amd_status cabana_amd_particle_migrate(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+    const cabana_amd_particle_field* positions,
+    const cabana_amd_particle_field* members, int32_t num_members,
+    int32_t min_halo_width, int32_t force_migrate, int64_t* num_particle,
+    int32_t* migrated, cabana_amd_stream stream );
+
+/* ----------------------------------------------------------------------- */
 /* Kernel timing (HIP events around every launch of this context)           */
 /* ----------------------------------------------------------------------- */
 

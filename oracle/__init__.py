@@ -99,6 +99,9 @@ def load():
         lib.orc_lcl_build.argtypes = [P(PField), c_int64, c_int64, P(c_double), P(c_double),
                                       P(c_double), P(c_int32), P(c_int64), P(c_int64)]
         lib.orc_permute.argtypes = [P(c_int64), c_int64, c_int64, P(PField)]
+        lib.orc_particle_migrate.restype = c_int64
+        lib.orc_particle_migrate.argtypes = [P(Mesh), P(c_double), c_int64, c_int, c_int,
+                                             P(c_int32)]
         _lib = lib
     return _lib
 
@@ -316,3 +319,18 @@ def permute(permutes, begin, end, field):
     perm = np.ascontiguousarray(permutes, dtype=np.int64)
     _check(load().orc_permute(perm.ctypes.data_as(P(c_int64)), begin, end,
                               ctypes.byref(field)), "permute")
+
+
+def particle_migrate(mesh, positions, min_halo_width, force=False):
+    """particleMigrate on one rank -> (migrated, shifted positions, keep
+    flags).  The positions array is not modified; keep is None when nothing
+    migrates."""
+    x = np.ascontiguousarray(positions, dtype=np.float64).copy()
+    n = x.shape[0]
+    keep = np.zeros(max(n, 1), dtype=np.int32)
+    kept = load().orc_particle_migrate(ctypes.byref(mesh.m), x.ctypes.data_as(P(c_double)), n,
+                                       int(min_halo_width), 1 if force else 0,
+                                       keep.ctypes.data_as(P(c_int32)))
+    if kept < 0:
+        return False, x, None
+    return True, x, keep[:n].astype(bool)

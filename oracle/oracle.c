@@ -1339,3 +1339,75 @@ int orc_permute( const int64_t* permutes, int64_t begin, int64_t end,
     free( scratch );
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Particle migration on one rank                                           */
+/* ------------------------------------------------------------------------ */
+
+/* particleMigrate (grid/src/Cabana_Grid_ParticleDistributor.hpp:330-352) on
+ * one rank, over row-major positions x[p][3].  migrateCount (:170-221):
+ * with force == 0 and no particle outside the ghosted corners moved in by
+ * min_halo_width cells, returns -1 and changes nothing.  Otherwise
+ * getMigrateDestinations (:87-148): the neighbour index of each dimension
+ * is 0 below the owned low corner, 2 above the owned high corner, 1 inside;
+ * getTopology (:48-57) on one rank gives this rank in every direction of a
+ * periodic dimension and -1 past a non-periodic boundary, so keep[p] = 0
+ * exactly when some non-periodic dimension is out; then periodic
+ * coordinates above the global high corner lose the extent, those below the
+ * low corner gain it.  Returns the number of particles kept. */
+int64_t orc_particle_migrate( const orc_mesh* m, double* x, int64_t n, int min_halo_width,
+                              int force, int32_t* keep )
+{
+    double own_lo[3], own_hi[3], band_lo[3], band_hi[3], extent[3];
+    for ( int d = 0; d < 3; ++d )
+    {
+        /* LocalMesh::lowCorner/highCorner, Cabana_Grid_LocalMesh.hpp:88-107 */
+        own_lo[d] = m->global_low[d] + m->cell_size[d] * 0;
+        own_hi[d] = m->global_low[d] + m->cell_size[d] * ( 0 + m->num_cell[d] );
+        double ghost_lo = own_lo[d] - m->halo * m->cell_size[d];
+        double ghost_hi = own_hi[d] + m->halo * m->cell_size[d];
+        band_lo[d] = ghost_lo + min_halo_width * m->cell_size[d];
+        band_hi[d] = ghost_hi - min_halo_width * m->cell_size[d];
+        /* GlobalMesh::extent, Cabana_Grid_GlobalMesh.hpp:153-157 */
+        extent[d] = m->global_high[d] - m->global_low[d];
+    }
+    if ( !force )
+    {
+        int64_t count = 0;
+        for ( int64_t p = 0; p < n; ++p )
+            for ( int d = 0; d < 3; ++d )
+                if ( x[3 * p + d] < band_lo[d] || x[3 * p + d] > band_hi[d] )
+                {
+                    count += 1;
+                    break;
+                }
+        if ( count == 0 )
+            return -1;
+    }
+    int64_t kept = 0;
+    for ( int64_t p = 0; p < n; ++p )
+    {
+        int stays = 1;
+        for ( int d = 0; d < 3; ++d )
+        {
+            double* xp = &x[3 * p + d];
+            int nid = 1;
+            if ( *xp < own_lo[d] )
+                nid = 0;
+            else if ( *xp > own_hi[d] )
+                nid = 2;
+            if ( nid != 1 && !m->periodic[d] )
+                stays = 0;
+            if ( m->periodic[d] )
+            {
+                if ( *xp > m->global_high[d] )
+                    *xp -= extent[d];
+                else if ( *xp < m->global_low[d] )
+                    *xp += extent[d];
+            }
+        }
+        keep[p] = stays;
+        kept += stays;
+    }
+    return kept;
+}

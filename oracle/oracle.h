@@ -173,6 +173,11 @@ int orc_lcl_build( const orc_pfield* positions, int64_t begin, int64_t end,
 int orc_permute( const int64_t* permutes, int64_t begin, int64_t end,
                  orc_pfield* field );
 
+/* particleMigrate on one rank over row-major positions x[n][3]: -1 when
+ * nothing migrates, else the kept count with keep[p] set (0 = removed). */
+int64_t orc_particle_migrate( const orc_mesh* m, double* x, int64_t n, int min_halo_width,
+                              int force, int32_t* keep );
+
 #ifdef __cplusplus
 }
 #endif
