@@ -17,6 +17,7 @@ c_double = ctypes.c_double
 c_void_p = ctypes.c_void_p
 
 FLOAT64, FLOAT32, INT32, INT64 = 0, 1, 2, 3
+NODE, CELL = 0, 1
 
 OK = 0
 ERROR_INVALID_ARGUMENT = 1
@@ -152,6 +153,9 @@ SIGNATURES = {
                                      P(ParticleField), c_void_p]),
     "cabana_amd_permute_members": (c_int32, [c_void_p, c_void_p, c_int64, c_int64,
                                              P(ParticleField), c_int32, c_void_p]),
+    "cabana_amd_bov_write_time_step": (c_int32, [c_void_p, P(Mesh), P(GridField), c_int32,
+                                                 ctypes.c_char_p, ctypes.c_char_p, c_int32,
+                                                 ctypes.c_double, c_int32, c_void_p]),
     "cabana_amd_particle_migrate": (c_int32, [c_void_p, P(Mesh), P(ParticleField),
                                               P(ParticleField), c_int32, c_int32, c_int32,
                                               P(c_int64), P(c_int32), c_void_p]),

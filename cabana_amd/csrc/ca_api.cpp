@@ -6,8 +6,11 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
 #include <map>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -1829,6 +1832,108 @@ extern "C" cabana_amd_status cabana_amd_permute_members(
             launch_permute_members( *ctx, f.data(), es.data(), num_fields, permutes, begin,
                                     end, ctx->scratch[SLOT_PERMUTE], s ),
             "permute members" );
+    return CABANA_AMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// BOV output (one rank)
+// ---------------------------------------------------------------------------
+
+extern "C" cabana_amd_status cabana_amd_bov_write_time_step(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, const cabana_amd_grid_field* grid,
+    int32_t entity, const char* label, const char* prefix, int32_t time_step_index,
+    double time, int32_t gather_array, cabana_amd_stream stream )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    if ( !grid || !label || !prefix )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "NULL argument" );
+    if ( entity != CABANA_AMD_NODE && entity != CABANA_AMD_CELL )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "entity must be NODE or CELL" );
+    MeshDev m;
+    CA_CHECK( make_mesh( ctx, mesh, m ) );
+    bool fp64 = true;
+    CA_CHECK( precision_of( ctx, grid->dtype, fp64 ) );
+    const bool node = entity == CABANA_AMD_NODE;
+    // A cell array spans num_cell + 2 * halo cells per dimension.
+    MeshDev mg = m;
+    if ( !node )
+        for ( int d = 0; d < 3; ++d )
+            mg.num_node[d] = m.num_cell[d] + 2 * m.halo;
+    GFieldDev g;
+    CA_CHECK( make_gfield( ctx, mg, *grid, -1, grid->dtype, false, "grid", g ) );
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    // writeTimeStep (Cabana_Grid_BovWriter.hpp:180-338) on one rank, the
+    // last block of every dimension: global extents are the cell counts
+    // (+1 for nodes); the written block starts at the first owned entity
+    // and covers all of them, including for nodes the closing node of a
+    // periodic dimension, which the width-0 node halo gather fills from its
+    // periodic owner first when gather_array is set.
+    int n[3];
+    for ( int d = 0; d < 3; ++d )
+        n[d] = m.num_cell[d] + ( node ? 1 : 0 );
+    bool any_periodic = false;
+    for ( int d = 0; d < 3; ++d )
+        any_periodic = any_periodic || m.periodic[d];
+    if ( node && gather_array && any_periodic )
+        CA_HIP( ctx, launch_halo_gather( *ctx, fp64, m, g, 0, s ), "halo gather" );
+    const int dofs = grid->extent[3];
+    const size_t es = fp64 ? 8 : 4;
+    const size_t count = (size_t)n[0] * n[1] * n[2] * dofs;
+    CA_HIP( ctx, ctx->ensure( SLOT_PERMUTE, es * count + 16 ), "scratch" );
+    CA_HIP( ctx, launch_bov_pack( *ctx, fp64, g, m.halo, n, ctx->scratch[SLOT_PERMUTE], s ),
+            "bov pack" );
+    std::vector<char> host( es * count );
+    CA_HIP( ctx,
+            hipMemcpyAsync( host.data(), ctx->scratch[SLOT_PERMUTE], es * count,
+                            hipMemcpyDeviceToHost, s ),
+            "hipMemcpyAsync" );
+    CA_HIP( ctx, hipStreamSynchronize( s ), "hipStreamSynchronize" );
+
+    std::stringstream file_name;
+    file_name << prefix << "_" << std::setfill( '0' ) << std::setw( 6 ) << time_step_index;
+    const std::string data_file_name = file_name.str() + ".dat";
+    {
+        std::ofstream data( data_file_name, std::ios::out | std::ios::binary | std::ios::trunc );
+        if ( !data )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "cannot open " + data_file_name );
+        data.write( host.data(), (std::streamsize)host.size() );
+        if ( !data )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                         "cannot write " + data_file_name );
+    }
+    // The header, field by field and in the stream formatting of
+    // Cabana_Grid_BovWriter.hpp:276-337.
+    const std::string header_file_name = file_name.str() + ".bov";
+    std::fstream header;
+    header.open( header_file_name, std::fstream::out );
+    if ( !header )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "cannot open " + header_file_name );
+    header << "TIME: " << time << std::endl;
+    header << "DATA_FILE: " << data_file_name << std::endl;
+    header << "DATA_SIZE: ";
+    for ( int d = 0; d < 3; ++d )
+        header << (long)n[d] << " ";
+    header << std::endl;
+    header << "DATA_FORMAT: " << ( fp64 ? "DOUBLE" : "FLOAT" ) << std::endl;
+    header << "VARIABLE: " << label << std::endl;
+    header << "DATA_ENDIAN: LITTLE" << std::endl;
+    header << "CENTERING: " << ( node ? "nodal" : "zonal" ) << std::endl;
+    header << "BRICK_ORIGIN: ";
+    for ( int d = 0; d < 3; ++d )
+        header << mesh->low_corner[d] << " ";
+    header << std::endl;
+    header << "BRICK_SIZE: ";
+    for ( int d = 0; d < 3; ++d )
+        header << m.num_cell[d] * mesh->cell_size[d] << " ";
+    header << std::endl;
+    header << "DATA_COMPONENTS: " << (long)dofs << std::endl;
+    header.close();
+    if ( !header )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "cannot write " + header_file_name );
     return CABANA_AMD_OK;
 }
 

@@ -765,6 +765,34 @@ __global__ __launch_bounds__( BLOCK ) void compact_index_kernel( const int32_t* 
         idx[rank[p]] = p;
 }
 
+// ---------------------------------------------------------------------------
+// BOV output
+// ---------------------------------------------------------------------------
+
+// The written block of a grid array in file order, x fastest after the
+// dofs: out[((k * n1 + j) * n0 + i) * dofs + l] = grid(lo + i, lo + j,
+// lo + k, l) (reorderView and the C-order subarray of
+// Cabana_Grid_BovWriter.hpp:82-152).
+template <class T>
+__global__ __launch_bounds__( BLOCK ) void bov_pack_kernel( GFieldDev g, int lo, int n0, int n1,
+                                                            int n2, T* out )
+{
+    const int dofs = g.ext[3];
+    const int64_t total = (int64_t)n0 * n1 * n2 * dofs;
+    const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( t >= total )
+        return;
+    int64_t r = t;
+    const int l = (int)( r % dofs );
+    r /= dofs;
+    const int i = (int)( r % n0 );
+    r /= n0;
+    const int j = (int)( r % n1 );
+    const int k = (int)( r / n1 );
+    out[t] = *gptr<const T>( g, ( lo + i ) * g.s[0] + ( lo + j ) * g.s[1] + ( lo + k ) * g.s[2] +
+                                    l * g.s[3] );
+}
+
 } // namespace
 
 hipError_t launch_halo_gather( Launcher& L, bool fp64, const MeshDev& m,
@@ -1220,6 +1248,23 @@ hipError_t launch_compact_index( Launcher& L, const int32_t* keep, const int64_t
     hipLaunchKernelGGL( compact_index_kernel, blocks_for( n ), dim3( BLOCK ), 0, s, keep, rank, n,
                         idx );
     L.after( "compact_index", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_bov_pack( Launcher& L, bool fp64, const GFieldDev& g, int lo, const int n[3],
+                            void* out, hipStream_t s )
+{
+    const int64_t total = (int64_t)n[0] * n[1] * n[2] * g.ext[3];
+    if ( total <= 0 )
+        return hipSuccess;
+    L.before( "bov_pack", s );
+    if ( fp64 )
+        hipLaunchKernelGGL( bov_pack_kernel<double>, blocks_for( total ), dim3( BLOCK ), 0, s, g,
+                            lo, n[0], n[1], n[2], static_cast<double*>( out ) );
+    else
+        hipLaunchKernelGGL( bov_pack_kernel<float>, blocks_for( total ), dim3( BLOCK ), 0, s, g,
+                            lo, n[0], n[1], n[2], static_cast<float*>( out ) );
+    L.after( "bov_pack", s );
     return hipGetLastError();
 }
 

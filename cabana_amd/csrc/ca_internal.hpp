@@ -472,6 +472,10 @@ hipError_t launch_migrate_count( Launcher& L, const PFieldDev& pos, int64_t n,
 hipError_t launch_migrate_shift( Launcher& L, const PFieldDev& pos, int64_t n,
                                  const MigrateBoundsDev& b, int32_t* keep,
                                  unsigned long long* dropped, hipStream_t s );
+// Block [lo, lo + n) of every dimension of a grid array, packed x fastest
+// after the dofs (the BOV data order) into out.
+hipError_t launch_bov_pack( Launcher& L, bool fp64, const GFieldDev& g, int lo, const int n[3],
+                            void* out, hipStream_t s );
 // idx[rank[p]] = p for the kept particles (rank: exclusive scan of keep).
 hipError_t launch_compact_index( Launcher& L, const int32_t* keep, const int64_t* rank,
                                  int64_t n, int64_t* idx, hipStream_t s );

@@ -95,6 +95,12 @@ class GlobalMesh:
 
 
 def createUniformGlobalMesh(low, high, cell_size):
+    """createUniformGlobalMesh(low, high, cell_size) with one cell size or one
+    per dimension, or with the number of cells per dimension (a sequence of
+    ints: cell size (high - low) / num_cell, Cabana_Grid_GlobalMesh.hpp:105-130)."""
+    if not isinstance(cell_size, (int, float)) and all(
+            isinstance(v, int) and not isinstance(v, bool) for v in cell_size):
+        cell_size = [(float(high[d]) - float(low[d])) / cell_size[d] for d in range(3)]
     return GlobalMesh(low, high, cell_size)
 
 
@@ -219,9 +225,12 @@ def createLocalMesh(local_grid):
 
 
 class ArrayLayout:
+    """Node or Cell array layout.  The interpolation and halo calls take node
+    arrays; cell arrays serve the BOV writer."""
+
     def __init__(self, local_grid, dofs, entity):
-        if not isinstance(entity, Node):
-            raise NotImplementedError("cabana_amd arrays are Node arrays")
+        if not isinstance(entity, (Node, Cell)):
+            raise NotImplementedError("cabana_amd arrays are Node or Cell arrays")
         self._lg = local_grid
         self._dofs = int(dofs)
         self.entity = entity
@@ -233,7 +242,7 @@ class ArrayLayout:
         return self._dofs
 
     def indexSpace(self, decomposition, index_type=None):
-        s = self._lg.indexSpace(decomposition, Node())
+        s = self._lg.indexSpace(decomposition, self.entity)
         return IndexSpace(list(s._min) + [0], list(s._max) + [self._dofs])
 
 
@@ -250,7 +259,8 @@ class Array:
         self.label = label
         self._layout = layout
         lg = layout.localGrid()
-        shape = [lg.totalNumCell(d) + 1 for d in range(3)] + [layout.dofsPerEntity()]
+        extra = 1 if isinstance(layout.entity, Node) else 0
+        shape = [lg.totalNumCell(d) + extra for d in range(3)] + [layout.dofsPerEntity()]
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         if memory_layout == "right":
@@ -313,7 +323,7 @@ class ArrayOp:
         (Cabana_Grid_Array.hpp:431-441)."""
         v = array.view()
         if isinstance(decomposition, Own):
-            s = array.layout().localGrid().indexSpace(Own(), Node())
+            s = array.layout().localGrid().indexSpace(Own(), array.layout().entity)
             v[s.min(0):s.max(0), s.min(1):s.max(1), s.min(2):s.max(2), :] = value
         else:
             v.fill_(value)
@@ -737,3 +747,42 @@ def particleMigrate(local_grid, positions, particles, min_halo_width, force_migr
         1 if force_migrate else 0, ctypes.byref(num), ctypes.byref(migrated),
         N.current_stream()), ctx)
     return bool(migrated.value), int(num.value)
+
+
+# ---------------------------------------------------------------------------
+# Grid output
+# ---------------------------------------------------------------------------
+
+
+class BovWriter:
+    """Experimental::BovWriter (Cabana_Grid_BovWriter.hpp) on one rank."""
+
+    @staticmethod
+    def writeTimeStep(*args, gather_array=True):
+        """writeTimeStep(prefix, time_step_index, time, array[, gather_array])
+        or writeTimeStep(time_step_index, time, array[, gather_array]) with
+        the prefix "grid_" + label (:180-338, :389-395).  Writes
+        <prefix>_<index:06d>.dat and .bov."""
+        if isinstance(args[0], str):
+            prefix, step, time, array = args[:4]
+            rest = args[4:]
+        else:
+            step, time, array = args[:3]
+            prefix = "grid_" + array.label
+            rest = args[3:]
+        if rest:
+            gather_array = rest[0]
+        layout = array.layout()
+        lg = layout.localGrid()
+        m = lg.mesh_descriptor()
+        g = grid_descriptor(array.view())
+        entity = N.NODE if isinstance(layout.entity, Node) else N.CELL
+        ctx = N.context()
+        N.check(N.load().cabana_amd_bov_write_time_step(
+            ctx, ctypes.byref(m), ctypes.byref(g), entity, array.label.encode(),
+            str(prefix).encode(), int(step), float(time), 1 if gather_array else 0,
+            N.current_stream()), ctx)
+
+
+class Experimental:
+    BovWriter = BovWriter

@@ -51,6 +51,16 @@ typedef enum
     CABANA_AMD_INT64 = 3
 } cabana_amd_dtype;
 
+/* Mesh entity of a grid array (Cabana_Grid_Types.hpp Node, Cell): a node
+ * array spans num_cell + 2 * halo + 1 nodes per dimension, a cell array
+ * num_cell + 2 * halo cells.  The interpolation entry points take node
+ * arrays. */
+typedef enum
+{
+    CABANA_AMD_NODE = 0,
+    CABANA_AMD_CELL = 1
+} cabana_amd_entity;
+
 /* Point-to-grid operators: the functors built by createScalarValueP2G,
  * createVectorValueP2G, createScalarGradientP2G, createVectorDivergenceP2G
  * and createTensorDivergenceP2G (Cabana_Grid_Interpolation.hpp:1324-1566). */
@@ -368,6 +378,27 @@ cabana_amd_status cabana_amd_permute_members(
     cabana_amd_context* ctx, const int64_t* permutes, int64_t begin, int64_t end,
     const cabana_amd_particle_field* fields, int32_t num_fields,
     cabana_amd_stream stream );
+
+/* ----------------------------------------------------------------------- */
+/* Grid output                                                              */
+/* ----------------------------------------------------------------------- */
+
+/* Experimental::BovWriter::writeTimeStep(prefix, time_step_index, time,
+ * array, gather_array) (grid/src/Cabana_Grid_BovWriter.hpp:180-338) on one
+ * rank: writes <prefix>_<index, 6 digits>.dat, the owned entities of every
+ * dimension (for nodes including the closing node of each dimension) x
+ * fastest with the dofs innermost, in the grid's precision, and the VisIt
+ * header <prefix>_<index>.bov (TIME, DATA_FILE, DATA_SIZE, DATA_FORMAT,
+ * VARIABLE = label, DATA_ENDIAN, CENTERING, BRICK_ORIGIN, BRICK_SIZE,
+ * DATA_COMPONENTS).  For a node array with a periodic dimension and
+ * gather_array set, a width-0 node halo gather first fills the closing
+ * nodes from their periodic owners (a side effect on the array, as there).
+ * Waits for the stream. */
+cabana_amd_status cabana_amd_bov_write_time_step(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+    const cabana_amd_grid_field* grid, int32_t entity, const char* label,
+    const char* prefix, int32_t time_step_index, double time,
+    int32_t gather_array, cabana_amd_stream stream );
 
 /* ----------------------------------------------------------------------- */
 /* Particle migration (one rank)                                             */

@@ -99,6 +99,8 @@ def load():
         lib.orc_lcl_build.argtypes = [P(PField), c_int64, c_int64, P(c_double), P(c_double),
                                       P(c_double), P(c_int32), P(c_int64), P(c_int64)]
         lib.orc_permute.argtypes = [P(c_int64), c_int64, c_int64, P(PField)]
+        lib.orc_bov_write.argtypes = [P(Mesh), P(Grid), c_int, ctypes.c_char_p, ctypes.c_char_p,
+                                      c_int, c_double, c_int]
         lib.orc_particle_migrate.restype = c_int64
         lib.orc_particle_migrate.argtypes = [P(Mesh), P(c_double), c_int64, c_int, c_int,
                                              P(c_int32)]
@@ -334,3 +336,12 @@ def particle_migrate(mesh, positions, min_halo_width, force=False):
     if kept < 0:
         return False, x, None
     return True, x, keep[:n].astype(bool)
+
+
+def bov_write(mesh, grid, entity, label, prefix, time_step_index, time, gather_array=True):
+    """BovWriter::writeTimeStep on one rank over an FP64 (i, j, k, dofs)
+    numpy grid (node or cell extents; entity "node" or "cell")."""
+    rc = load().orc_bov_write(ctypes.byref(mesh.m), ctypes.byref(gridfield(grid)),
+                              0 if entity == "node" else 1, label.encode(), prefix.encode(),
+                              int(time_step_index), float(time), 1 if gather_array else 0)
+    _check(rc, "bov write")

@@ -10,6 +10,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1410,4 +1411,74 @@ int64_t orc_particle_migrate( const orc_mesh* m, double* x, int64_t n, int min_h
         kept += stays;
     }
     return kept;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BOV output on one rank                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Experimental::BovWriter::writeTimeStep (grid/src/Cabana_Grid_BovWriter.hpp
+ * :180-338) on one rank.  entity 0 = Node (grid extents num_node), 1 = Cell
+ * (num_cell + 2h).  Global extents are the cell counts, +1 for nodes
+ * (:196-203); the owned extents of the single (last) block are the same
+ * (:210-225); a node array with a periodic dimension is first gathered with
+ * a width-0 halo (:227-240).  The data file holds grid(h + i, h + j, h + k, l)
+ * in (k, j, i, l) order (reorderView :140-152, C-order subarray :82-122);
+ * the header lines are those of :276-337, doubles in the stream's default
+ * format, which is printf's %g. */
+int orc_bov_write( const orc_mesh* m, orc_grid* grid, int entity, const char* label,
+                   const char* prefix, int time_step_index, double time, int gather_array )
+{
+    const int node = entity == 0;
+    long ext[3];
+    for ( int d = 0; d < 3; ++d )
+    {
+        ext[d] = m->num_cell[d] + ( node ? 1 : 0 );
+        int want = node ? m->num_node[d] : m->num_cell[d] + 2 * m->halo;
+        if ( grid->ext[d] != want )
+            return 1;
+    }
+    if ( node && gather_array )
+        for ( int d = 0; d < 3; ++d )
+            if ( m->periodic[d] )
+            {
+                if ( orc_halo_gather( m, grid, 0 ) )
+                    return 1;
+                break;
+            }
+    char base[4096];
+    snprintf( base, sizeof( base ), "%s_%06d", prefix, time_step_index );
+    char data_name[4200], header_name[4200];
+    snprintf( data_name, sizeof( data_name ), "%s.dat", base );
+    snprintf( header_name, sizeof( header_name ), "%s.bov", base );
+    FILE* f = fopen( data_name, "wb" );
+    if ( !f )
+        return 2;
+    const int h = m->halo;
+    for ( long k = 0; k < ext[2]; ++k )
+        for ( long j = 0; j < ext[1]; ++j )
+            for ( long i = 0; i < ext[0]; ++i )
+                for ( int l = 0; l < grid->ext[3]; ++l )
+                {
+                    double v = *gd_ptr( grid, h + (int)i, h + (int)j, h + (int)k, l );
+                    fwrite( &v, sizeof( double ), 1, f );
+                }
+    fclose( f );
+    f = fopen( header_name, "w" );
+    if ( !f )
+        return 2;
+    fprintf( f, "TIME: %g\n", time );
+    fprintf( f, "DATA_FILE: %s\n", data_name );
+    fprintf( f, "DATA_SIZE: %ld %ld %ld \n", ext[0], ext[1], ext[2] );
+    fprintf( f, "DATA_FORMAT: DOUBLE\n" );
+    fprintf( f, "VARIABLE: %s\n", label );
+    fprintf( f, "DATA_ENDIAN: LITTLE\n" );
+    fprintf( f, "CENTERING: %s\n", node ? "nodal" : "zonal" );
+    fprintf( f, "BRICK_ORIGIN: %g %g %g \n", m->global_low[0], m->global_low[1],
+             m->global_low[2] );
+    fprintf( f, "BRICK_SIZE: %g %g %g \n", m->num_cell[0] * m->cell_size[0],
+             m->num_cell[1] * m->cell_size[1], m->num_cell[2] * m->cell_size[2] );
+    fprintf( f, "DATA_COMPONENTS: %d\n", grid->ext[3] );
+    fclose( f );
+    return 0;
 }

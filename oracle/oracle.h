@@ -178,6 +178,11 @@ int orc_permute( const int64_t* permutes, int64_t begin, int64_t end,
 int64_t orc_particle_migrate( const orc_mesh* m, double* x, int64_t n, int min_halo_width,
                               int force, int32_t* keep );
 
+/* BovWriter::writeTimeStep on one rank (FP64 grids); entity 0 = Node,
+ * 1 = Cell.  Returns 0 on success. */
+int orc_bov_write( const orc_mesh* m, orc_grid* grid, int entity, const char* label,
+                   const char* prefix, int time_step_index, double time, int gather_array );
+
 #ifdef __cplusplus
 }
 #endif
