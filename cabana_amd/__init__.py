@@ -10,7 +10,8 @@ from .grid import (  # noqa: F401
     createLocalGrid, createLocalMesh, createScalarGradientG2P, createScalarGradientP2G,
     createScalarValueG2P, createScalarValueP2G, createTensorDivergenceP2G,
     createUniformGlobalMesh, createVectorDivergenceG2P, createVectorDivergenceP2G,
-    createVectorGradientG2P, createVectorValueG2P, createVectorValueP2G, g2p, g2p_adjoint,
-    grid_descriptor, p2g, p2g_adjoint, particleMigrate, particle_descriptor)
+    createVectorGradientG2P, createVectorValueG2P, createVectorValueP2G, g2p, g2p2g, g2p_adjoint,
+    grid_descriptor, p2g, p2g_adjoint, particleMigrate, particle_descriptor, profile_enable,
+    profile_report, profile_reset)
 from .linkedcell import LinkedCellList, createLinkedCellList, permute  # noqa: F401
 from . import autograd  # noqa: F401,E402

@@ -135,6 +135,9 @@ SIGNATURES = {
                                           c_void_p]),
     "cabana_amd_g2p": (c_int32, [c_void_p, P(Mesh), c_int32, P(ParticleField), c_int32,
                                  P(G2PTerm), c_int32, P(Binning), c_void_p]),
+    "cabana_amd_g2p2g": (c_int32, [c_void_p, P(Mesh), c_int32, P(ParticleField), c_int32,
+                                   P(G2PTerm), c_int32, P(P2GTerm), c_int32, P(Binning),
+                                   c_void_p]),
     "cabana_amd_p2g_adjoint": (c_int32, [c_void_p, P(Mesh), c_int32, P(ParticleField),
                                          c_int32, P(P2GAdjointTerm), c_int32,
                                          P(ParticleField), P(Binning), c_void_p]),

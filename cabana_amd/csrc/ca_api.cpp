@@ -70,8 +70,9 @@ struct cabana_amd_context : public Launcher
     unsigned long long* bad = nullptr;     // device counter (binning)
 
     // Reusable scratch.
-    void* scratch[5] = { nullptr, nullptr, nullptr, nullptr, nullptr };
-    size_t scratch_bytes[5] = { 0, 0, 0, 0, 0 };
+    static constexpr int NUM_SLOTS = 6;
+    void* scratch[NUM_SLOTS] = {};
+    size_t scratch_bytes[NUM_SLOTS] = {};
 
     // Brick part tables of binnings that carry a build serial, each in its
     // own allocation (the general scratch slots move when they grow); the
@@ -191,7 +192,8 @@ enum Slot
     SLOT_CURSOR = 1,
     SLOT_SCAN = 2,
     SLOT_PERMUTE = 3,
-    SLOT_TILES = 4
+    SLOT_TILES = 4,
+    SLOT_SNAPSHOT = 5 // copy of a grid that one pass both reads and adds into
 };
 
 cabana_amd_status fail( cabana_amd_context* ctx, cabana_amd_status st,
@@ -531,6 +533,44 @@ hipError_t run_scatter_cells( cabana_amd_context* ctx, int order, bool fp64,
     default:
         return launch_scatter_cells_o3( *ctx, fp64, plan, a, t, lds, s );
     }
+}
+
+hipError_t run_multi_term( cabana_amd_context* ctx, int order, bool fp64,
+                          const MultiTermArgs& a, hipStream_t s )
+{
+    switch ( order )
+    {
+    case 0:
+        return launch_multi_term_o0( *ctx, fp64, a, s );
+    case 1:
+        return launch_multi_term_o1( *ctx, fp64, a, s );
+    case 2:
+        return launch_multi_term_o2( *ctx, fp64, a, s );
+    case 3:
+        return launch_multi_term_o3( *ctx, fp64, a, s );
+    case 4:
+        return launch_multi_term_o4( *ctx, fp64, a, s );
+    case 5:
+        return launch_multi_term_o5( *ctx, fp64, a, s );
+    default:
+        return launch_multi_term_o6( *ctx, fp64, a, s );
+    }
+}
+
+// Whether a pass over these particles can run the cell-tiled kernels.
+bool tiled_call( const cabana_amd_binning* bins, int order )
+{
+    return bins && bins->counts && bins->offsets && order >= 1 && order <= 3;
+}
+
+// Index of the first term of a form, or -1.
+template <class Term>
+int find_form( const Term* T, int n, int form, const std::vector<bool>& taken )
+{
+    for ( int i = 0; i < n; ++i )
+        if ( !taken[i] && T[i].form == form )
+            return i;
+    return -1;
 }
 
 size_t a16( size_t b ) { return ( b + 15 ) & ~(size_t)15; }
@@ -1065,7 +1105,7 @@ extern "C" cabana_amd_status cabana_amd_context_destroy( cabana_amd_context* ctx
     ctx->drain();
     for ( auto e : ctx->pool )
         hipEventDestroy( e );
-    for ( int i = 0; i < 5; ++i )
+    for ( int i = 0; i < cabana_amd_context::NUM_SLOTS; ++i )
         if ( ctx->scratch[i] )
             hipFree( ctx->scratch[i] );
     for ( auto& pt : ctx->tables )
@@ -1168,23 +1208,56 @@ cabana_amd_status p2g_impl( cabana_amd_context* ctx, const cabana_amd_mesh* mesh
     hipStream_t s = static_cast<hipStream_t>( stream );
     if ( n > 0 )
     {
-        const bool mpm = num_terms == 3 && T[0].form == FORM_SV &&
-                         T[1].form == FORM_VV && T[2].form == FORM_TD;
-        const bool mm = num_terms == 2 && T[0].form == FORM_SV && T[1].form == FORM_VV;
-        if ( num_terms == 1 || mpm || mm )
+        // The fused plans, whatever the order of the terms: (ScalarValue,
+        // VectorValue, TensorDivergence) or (ScalarValue, VectorValue).
+        std::vector<bool> taken( num_terms, false );
+        int fused[3] = { -1, -1, -1 };
+        int nfused = 0;
+        const int iSV = find_form( T, num_terms, FORM_SV, taken );
+        const int iVV = find_form( T, num_terms, FORM_VV, taken );
+        const int iTD = find_form( T, num_terms, FORM_TD, taken );
+        if ( num_terms > 1 && iSV >= 0 && iVV >= 0 )
         {
-            a.nterms = num_terms;
+            fused[0] = iSV;
+            fused[1] = iVV;
+            nfused = 2;
+            if ( iTD >= 0 )
+                fused[nfused++] = iTD;
+            for ( int k = 0; k < nfused; ++k )
+                taken[fused[k]] = true;
+        }
+        const int rest = num_terms - nfused;
+        if ( num_terms > 1 && rest > 0 && !tiled_call( bins, order ) )
+        {
+            // Untiled: every term in one pass (the spline once per particle).
+            MultiTermArgs ma;
+            std::memset( &ma, 0, sizeof( ma ) );
+            ma.mesh = m;
+            ma.points = a.points;
+            ma.begin = 0;
+            ma.end = n;
+            ma.n_p2g = num_terms;
             for ( int i = 0; i < num_terms; ++i )
-                a.t[i] = T[i];
-            CA_CHECK( scatter_pass( ctx, order, fp64,
-                                    mpm ? PLAN_MPM_P2G
-                                        : ( mm ? PLAN_MASS_MOMENTUM : PLAN_SINGLE ),
-                                    a, n, bins, s, fixed, fixed_bytes ) );
+                ma.s[i] = multi_term( T[i] );
+            ma.skipped = ctx->skipped;
+            CA_HIP( ctx, run_multi_term( ctx, order, fp64, ma, s ), "p2g terms" );
         }
         else
         {
+            if ( nfused > 0 )
+            {
+                ScatterArgs b = a;
+                b.nterms = nfused;
+                for ( int k = 0; k < nfused; ++k )
+                    b.t[k] = T[fused[k]];
+                CA_CHECK( scatter_pass( ctx, order, fp64,
+                                        nfused == 3 ? PLAN_MPM_P2G : PLAN_MASS_MOMENTUM, b, n,
+                                        bins, s, fixed, fixed_bytes ) );
+            }
             for ( int i = 0; i < num_terms; ++i )
             {
+                if ( taken[i] )
+                    continue;
                 ScatterArgs b = a;
                 b.nterms = 1;
                 b.t[0] = T[i];
@@ -1251,6 +1324,68 @@ cabana_amd_status fail_with( cabana_amd_context* ctx, cabana_amd_status st, cons
 }
 } // namespace cabana_amd
 
+namespace
+{
+// The passes of a G2P call.  (VectorValue, VectorGradient) from one grid
+// into distinct fields is the fused plan, in either order; without tiles
+// any other set runs in one pass; with tiles the other terms run one pass
+// each.
+cabana_amd_status g2p_passes( cabana_amd_context* ctx, int order, bool fp64, const GatherArgs& a,
+                              const GatherTermDev* T, int num_terms, int64_t n,
+                              const cabana_amd_binning* bins, hipStream_t s )
+{
+    std::vector<bool> taken( num_terms, false );
+    int pair[2] = { -1, -1 };
+    for ( int i = 0; i < num_terms && pair[0] < 0; ++i )
+        for ( int j = 0; j < num_terms; ++j )
+            if ( T[i].form == FORM_VV && T[j].form == FORM_TDT && same_grid( T[i].grid, T[j].grid ) &&
+                 !pfields_overlap( T[i].pout, T[j].pout, n, fp64 ? 8 : 4 ) )
+            {
+                pair[0] = i;
+                pair[1] = j;
+                break;
+            }
+    const int nfused = pair[0] >= 0 ? 2 : 0;
+    if ( nfused )
+        taken[pair[0]] = taken[pair[1]] = true;
+    if ( num_terms > 1 && num_terms > nfused && !tiled_call( bins, order ) )
+    {
+        MultiTermArgs ma;
+        std::memset( &ma, 0, sizeof( ma ) );
+        ma.mesh = a.mesh;
+        ma.points = a.points;
+        ma.begin = 0;
+        ma.end = n;
+        ma.n_g2p = num_terms;
+        for ( int i = 0; i < num_terms; ++i )
+            ma.g[i] = multi_term( T[i] );
+        ma.skipped = ctx->skipped;
+        CA_HIP( ctx, run_multi_term( ctx, order, fp64, ma, s ), "g2p terms" );
+        return CABANA_AMD_OK;
+    }
+    if ( num_terms == 1 || nfused )
+    {
+        GatherArgs b = a;
+        b.nterms = num_terms == 1 ? 1 : 2;
+        b.t[0] = T[num_terms == 1 ? 0 : pair[0]];
+        if ( nfused )
+            b.t[1] = T[pair[1]];
+        CA_CHECK( gather_pass( ctx, order, fp64, nfused ? PLAN_MPM_G2P : PLAN_SINGLE, b, n, bins,
+                               s ) );
+    }
+    for ( int i = 0; i < num_terms && num_terms > 1; ++i )
+    {
+        if ( taken[i] )
+            continue;
+        GatherArgs b = a;
+        b.nterms = 1;
+        b.t[0] = T[i];
+        CA_CHECK( gather_pass( ctx, order, fp64, PLAN_SINGLE, b, n, bins, s ) );
+    }
+    return CABANA_AMD_OK;
+}
+} // namespace
+
 extern "C" cabana_amd_status cabana_amd_g2p(
     cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
     const cabana_amd_particle_field* points, int32_t num_terms,
@@ -1306,24 +1441,163 @@ extern "C" cabana_amd_status cabana_amd_g2p(
     }
     if ( n <= 0 )
         return CABANA_AMD_OK;
-    const bool pair = num_terms == 2 && T[0].form == FORM_VV &&
-                      T[1].form == FORM_TDT &&
-                      same_grid( T[0].grid, T[1].grid ) &&
-                      !pfields_overlap( T[0].pout, T[1].pout, n, fp64 ? 8 : 4 );
-    if ( num_terms == 1 || pair )
+    return g2p_passes( ctx, order, fp64, a, T.data(), num_terms, n, bins, s );
+}
+
+// ---------------------------------------------------------------------------
+// G2P followed by P2G in one call
+// ---------------------------------------------------------------------------
+
+namespace
+{
+// Bytes from a grid's first element to one past its last.
+size_t grid_span( const GFieldDev& g, bool fp64 )
+{
+    int64_t last = 0;
+    for ( int d = 0; d < 4; ++d )
+        last += (int64_t)( g.ext[d] - 1 ) * g.s[d];
+    return (size_t)( last + 1 ) * ( fp64 ? 8 : 4 );
+}
+
+bool grids_overlap( const GFieldDev& x, const GFieldDev& y, bool fp64 )
+{
+    return x.data < y.data + grid_span( y, fp64 ) && y.data < x.data + grid_span( x, fp64 );
+}
+} // namespace
+
+extern "C" cabana_amd_status cabana_amd_g2p2g(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
+    const cabana_amd_particle_field* points, int32_t num_g2p_terms,
+    const cabana_amd_g2p_term* g2p_terms, int32_t num_p2g_terms,
+    const cabana_amd_p2g_term* p2g_terms, int32_t halo_width,
+    const cabana_amd_binning* bins, cabana_amd_stream stream )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    if ( !points || num_g2p_terms < 1 || num_g2p_terms > MAX_CALL_TERMS || !g2p_terms ||
+         num_p2g_terms < 1 || num_p2g_terms > MAX_CALL_TERMS || !p2g_terms )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "need points, 1..16 G2P terms and 1..16 P2G terms" );
+    if ( tiled_call( bins, order ) )
     {
-        a.nterms = num_terms;
-        for ( int i = 0; i < num_terms; ++i )
-            a.t[i] = T[i];
-        return gather_pass( ctx, order, fp64, pair ? PLAN_MPM_G2P : PLAN_SINGLE,
-                            a, n, bins, s );
+        // Cell-tiled: the G2P passes, then the P2G passes.  Every read of a
+        // grid precedes every addition into it, as in the reference's
+        // g2p2g with a duplicated (Serial, OpenMP) scatter view.
+        CA_CHECK( cabana_amd_g2p( ctx, mesh, order, points, num_g2p_terms, g2p_terms,
+                                  halo_width, bins, stream ) );
+        return cabana_amd_p2g( ctx, mesh, order, points, num_p2g_terms, p2g_terms, halo_width,
+                               bins, stream );
     }
-    for ( int i = 0; i < num_terms; ++i )
+    MeshDev m;
+    CA_CHECK( make_mesh( ctx, mesh, m ) );
+    CA_CHECK( check_order( ctx, order ) );
+    const int w = resolve_width( m, halo_width );
+    CA_CHECK( check_width( ctx, m, w ) );
+    const int64_t n = points->num_particle;
+    CA_CHECK( validate_bins( ctx, bins, n ) );
+    bool fp64 = true;
+    CA_CHECK( precision_of( ctx, g2p_terms[0].values.dtype, fp64 ) );
+    const int rdt = fp64 ? CABANA_AMD_FLOAT64 : CABANA_AMD_FLOAT32;
+    MultiTermArgs a;
+    std::memset( &a, 0, sizeof( a ) );
+    a.mesh = m;
+    a.skipped = ctx->skipped;
+    a.begin = 0;
+    a.end = n;
+    a.n_g2p = num_g2p_terms;
+    a.n_p2g = num_p2g_terms;
+    CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, n, false, "points", a.points ) );
+    for ( int i = 0; i < num_g2p_terms; ++i )
     {
-        GatherArgs b = a;
-        b.nterms = 1;
-        b.t[0] = T[i];
-        CA_CHECK( gather_pass( ctx, order, fp64, PLAN_SINGLE, b, n, bins, s ) );
+        const auto& t = g2p_terms[i];
+        if ( t.op < 0 || t.op > 4 )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "unknown G2P operator" );
+        const int form = g2p_form( t.op );
+        a.g[i].form = form;
+        a.g[i].mult = t.multiplier;
+        CA_CHECK( make_pfield( ctx, t.values, form_pcomp( form ), rdt, n, false, "values",
+                               a.g[i].f ) );
+        CA_CHECK( make_gfield( ctx, m, t.grid, form_gdof( form ), rdt, false, "grid",
+                               a.g[i].grid ) );
+    }
+    for ( int i = 0; i < num_p2g_terms; ++i )
+    {
+        const auto& t = p2g_terms[i];
+        if ( t.op < 0 || t.op > 4 )
+            return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "unknown P2G operator" );
+        const int form = p2g_form( t.op );
+        a.s[i].form = form;
+        a.s[i].mult = t.multiplier;
+        CA_CHECK( make_pfield( ctx, t.values, form_pcomp( form ), rdt, n, false, "values",
+                               a.s[i].f ) );
+        CA_CHECK( make_gfield( ctx, m, t.grid, form_gdof( form ), rdt, false, "grid",
+                               a.s[i].grid ) );
+    }
+    hipStream_t s = static_cast<hipStream_t>( stream );
+    // Halo::gather of every distinct G2P grid (Cabana_Grid_Interpolation
+    // Performance.cpp:140-141 gathers before the point loop).
+    for ( int i = 0; i < num_g2p_terms; ++i )
+    {
+        bool seen = false;
+        for ( int j = 0; j < i; ++j )
+            seen = seen || a.g[j].grid.data == a.g[i].grid.data;
+        if ( !seen )
+            CA_HIP( ctx, launch_halo_gather( *ctx, fp64, m, a.g[i].grid, w, s ), "halo gather" );
+    }
+    if ( n > 0 )
+    {
+        // A G2P grid that a P2G term also adds into is read from a copy
+        // taken after the gather, so that every particle reads the values
+        // the call started from.
+        size_t need = 0;
+        std::vector<int> copied( num_g2p_terms, -1 );
+        for ( int i = 0; i < num_g2p_terms; ++i )
+        {
+            bool hit = false;
+            for ( int j = 0; j < num_p2g_terms; ++j )
+                hit = hit || grids_overlap( a.g[i].grid, a.s[j].grid, fp64 );
+            if ( !hit )
+                continue;
+            for ( int j = 0; j < i; ++j )
+                if ( copied[j] >= 0 && a.g[j].grid.data == a.g[i].grid.data )
+                    copied[i] = copied[j];
+            if ( copied[i] < 0 )
+            {
+                copied[i] = (int)need;
+                need = a16( need + grid_span( a.g[i].grid, fp64 ) );
+            }
+        }
+        if ( need > 0 )
+        {
+            CA_HIP( ctx, ctx->ensure( SLOT_SNAPSHOT, need ), "scratch" );
+            char* base = static_cast<char*>( ctx->scratch[SLOT_SNAPSHOT] );
+            for ( int i = 0; i < num_g2p_terms; ++i )
+            {
+                if ( copied[i] < 0 )
+                    continue;
+                bool first = true;
+                for ( int j = 0; j < i; ++j )
+                    first = first && !( copied[j] == copied[i] );
+                if ( first )
+                    CA_HIP( ctx,
+                            hipMemcpyAsync( base + copied[i], a.g[i].grid.data,
+                                            grid_span( a.g[i].grid, fp64 ),
+                                            hipMemcpyDeviceToDevice, s ),
+                            "hipMemcpyAsync" );
+                a.g[i].grid.data = base + copied[i];
+            }
+        }
+        CA_HIP( ctx, run_multi_term( ctx, order, fp64, a, s ), "g2p2g" );
+    }
+    // Halo::scatter(Sum) of every distinct P2G grid.
+    for ( int i = 0; i < num_p2g_terms; ++i )
+    {
+        bool seen = false;
+        for ( int j = 0; j < i; ++j )
+            seen = seen || a.s[j].grid.data == a.s[i].grid.data;
+        if ( !seen )
+            CA_HIP( ctx, launch_halo_scatter_sum( *ctx, fp64, m, a.s[i].grid, w, s ),
+                    "halo scatter" );
     }
     return CABANA_AMD_OK;
 }

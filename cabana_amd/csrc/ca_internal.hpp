@@ -280,6 +280,37 @@ struct GatherArgs
     unsigned long long* skipped;
 };
 
+// Every term of a call in one pass over the particles, for any set of
+// operators: the spline is evaluated once per particle, then the G2P terms
+// run in call order (each adds into its particle field) and after them the
+// P2G terms (each reads its particle field, which a G2P term of the same
+// particle may just have updated, and adds into its grid with atomics).
+// One term of such a pass: its particle field (output of a G2P term, input
+// of a P2G term), grid, multiplier and form.
+struct MultiTermDev
+{
+    PFieldDev f;
+    GFieldDev grid;
+    double mult;
+    int32_t form;
+};
+struct MultiTermArgs
+{
+    MeshDev mesh;
+    PFieldDev points;
+    int64_t begin;
+    int64_t end;
+    int32_t n_g2p;
+    int32_t n_p2g;
+    MultiTermDev g[MAX_CALL_TERMS];
+    MultiTermDev s[MAX_CALL_TERMS];
+    unsigned long long* skipped;
+};
+// Kernel arguments are limited to 4 KiB.
+static_assert( sizeof( MultiTermArgs ) <= 4000, "MultiTermArgs exceeds the kernel argument space" );
+inline MultiTermDev multi_term( const GatherTermDev& t ) { return { t.pout, t.grid, t.mult, t.form }; }
+inline MultiTermDev multi_term( const ScatterTermDev& t ) { return { t.pin, t.grid, t.mult, t.form }; }
+
 // Brick tiling of cell-sorted particles.  A brick is tile[0] x tile[1] x
 // tile[2] mesh cells; its particles are the tile[0] * tile[1] cell runs along
 // the last dimension (contiguous in the cell-sorted order).  Bricks are cut
@@ -349,7 +380,9 @@ enum Plan : int32_t
                                        const GatherArgs& a, hipStream_t s );   \
     hipError_t launch_scatter_atomic_##SUFFIX( Launcher& L, bool fp64,         \
                                                int plan, const ScatterArgs& a, \
-                                               hipStream_t s );
+                                               hipStream_t s );                \
+    hipError_t launch_multi_term_##SUFFIX( Launcher& L, bool fp64,             \
+                                           const MultiTermArgs& a, hipStream_t s );
 #define CABANA_AMD_DECLARE_TILED( SUFFIX )                                     \
     hipError_t launch_gather_tiled_##SUFFIX( Launcher& L, bool fp64, int plan, \
                                              const GatherArgs& a,              \

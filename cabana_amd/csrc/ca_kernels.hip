@@ -220,6 +220,136 @@ __global__ __launch_bounds__( BLOCK ) void scatter_atomic_kernel( ScatterArgs a 
 }
 
 // ---------------------------------------------------------------------------
+// Any set of terms in one pass (the reference benchmark's fused_p2g,
+// fused_g2p and g2p2g, Cabana_Grid_InterpolationPerformance.cpp:25-174)
+// ---------------------------------------------------------------------------
+
+// One G2P term of one particle: pout += mult * dB/dP over the stencil.
+template <class Real, int FORM>
+__device__ __forceinline__ void gather_term( const MultiTermDev& term, int64_t p,
+                                             const Spline3D<ORDER, Real, false>& s )
+{
+    constexpr int NP = FormTraits<FORM>::NP;
+    constexpr int NG = FormTraits<FORM>::NG;
+    const GFieldDev& g = term.grid;
+    const Real* g0 = grid_at<Real>( g, s.d[0].base, s.d[1].base, s.d[2].base );
+    Real acc[NP];
+#pragma unroll
+    for ( int c = 0; c < NP; ++c )
+        acc[c] = Real( 0 );
+#pragma unroll
+    for ( int i = 0; i < K; ++i )
+#pragma unroll
+        for ( int j = 0; j < K; ++j )
+#pragma unroll
+            for ( int k = 0; k < K; ++k )
+            {
+                const auto n = node_weights<ORDER, Real, false, false>( s, i, j, k );
+                const Real* gn = g0 + i * g.s[0] + j * g.s[1] + k * g.s[2];
+                Real G[NG];
+#pragma unroll
+                for ( int d = 0; d < NG; ++d )
+                    G[d] = gn[d * g.s[3]];
+                form_gather<FORM>( G, n, acc );
+            }
+    const Real m = (Real)term.mult;
+    const uint32_t b = term.f.base( p );
+#pragma unroll
+    for ( int c = 0; c < NP; ++c )
+    {
+        Real* o = term.f.template ptr<Real>( b, c );
+        *o += m * acc[c];
+    }
+}
+
+// One P2G term of one particle: grid += dB/dG of mult * P over the stencil.
+template <class Real, int FORM>
+__device__ __forceinline__ void scatter_term( const MultiTermDev& term, int64_t p,
+                                              const Spline3D<ORDER, Real, false>& s )
+{
+    constexpr int NP = FormTraits<FORM>::NP;
+    constexpr int NG = FormTraits<FORM>::NG;
+    const Real m = (Real)term.mult;
+    const uint32_t b = term.f.base( p );
+    Real Pm[NP];
+#pragma unroll
+    for ( int c = 0; c < NP; ++c )
+        Pm[c] = *term.f.template ptr<const Real>( b, c ) * m;
+    const GFieldDev& g = term.grid;
+    Real* g0 = gptr<Real>( g, s.d[0].base * g.s[0] + s.d[1].base * g.s[1] +
+                                  s.d[2].base * g.s[2] );
+#pragma unroll
+    for ( int i = 0; i < K; ++i )
+#pragma unroll
+        for ( int j = 0; j < K; ++j )
+#pragma unroll
+            for ( int k = 0; k < K; ++k )
+            {
+                const auto n = node_weights<ORDER, Real, false, false>( s, i, j, k );
+                Real G[NG];
+                form_scatter<FORM>( Pm, n, G );
+                Real* gn = g0 + i * g.s[0] + j * g.s[1] + k * g.s[2];
+#pragma unroll
+                for ( int d = 0; d < NG; ++d )
+                    atomic_add( gn + d * g.s[3], G[d] );
+            }
+}
+
+template <class Real>
+__global__ __launch_bounds__( BLOCK ) void multi_term_kernel( MultiTermArgs a )
+{
+    const int64_t p = a.begin + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( p >= a.end )
+        return;
+    double px[3];
+    load_position( a.points, p, px );
+    Spline3D<ORDER, Real, false> s;
+    if ( !eval_spline( a.mesh, px, s ) )
+    {
+        atomicAdd( a.skipped, 1ull );
+        return;
+    }
+    // The term index is uniform across the wave, so each switch is a
+    // scalar branch.
+    for ( int t = 0; t < a.n_g2p; ++t )
+    {
+        switch ( a.g[t].form )
+        {
+#define CA_CASE( F )                                                           \
+    case F:                                                                    \
+        gather_term<Real, F>( a.g[t], p, s );                                  \
+        break;
+            CA_CASE( FORM_SV )
+            CA_CASE( FORM_VV )
+            CA_CASE( FORM_SG )
+            CA_CASE( FORM_VD )
+            CA_CASE( FORM_TDT )
+#undef CA_CASE
+        default:
+            break;
+        }
+    }
+    for ( int t = 0; t < a.n_p2g; ++t )
+    {
+        switch ( a.s[t].form )
+        {
+#define CA_CASE( F )                                                           \
+    case F:                                                                    \
+        scatter_term<Real, F>( a.s[t], p, s );                                 \
+        break;
+            CA_CASE( FORM_SV )
+            CA_CASE( FORM_VV )
+            CA_CASE( FORM_SG )
+            CA_CASE( FORM_VD )
+            CA_CASE( FORM_TD )
+#undef CA_CASE
+        default:
+            break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
 
@@ -386,7 +516,26 @@ hipError_t scatter_atomic_typed( Launcher& L, int plan, const ScatterArgs& a,
     return e;
 }
 
+template <class Real>
+hipError_t multi_term_typed( Launcher& L, const MultiTermArgs& a, hipStream_t s )
+{
+    if ( a.end <= a.begin )
+        return hipSuccess;
+    const char* name = a.n_g2p == 0 ? "p2g_terms" : ( a.n_p2g == 0 ? "g2p_terms" : "g2p2g" );
+    L.before( name, s );
+    hipLaunchKernelGGL( multi_term_kernel<Real>, blocks_for( a.end - a.begin ), dim3( BLOCK ), 0,
+                        s, a );
+    L.after( name, s );
+    return hipGetLastError();
+}
+
 } // namespace
+
+hipError_t CA_CAT( launch_multi_term_, CA_SUFFIX )( Launcher& L, bool fp64,
+                                                   const MultiTermArgs& a, hipStream_t s )
+{
+    return fp64 ? multi_term_typed<double>( L, a, s ) : multi_term_typed<float>( L, a, s );
+}
 
 hipError_t CA_CAT( launch_gather_, CA_SUFFIX )( Launcher& L, bool fp64,
                                                int plan, const GatherArgs& a,

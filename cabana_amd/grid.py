@@ -652,6 +652,49 @@ def g2p(array, halo, points, num_point, spline, functor, bins=None):
                                     N.current_stream()), ctx)
 
 
+def _terms(kind, functors, arrays, num_point):
+    terms = (N.P2GTerm if kind == "p2g" else N.G2PTerm) * len(functors)
+    out = terms()
+    for i, (f, a) in enumerate(zip(functors, arrays)):
+        if f.direction != kind:
+            raise TypeError("expected %s functors" % kind.upper())
+        out[i].op = f.op
+        out[i].multiplier = f._multiplier
+        out[i].values = _values_descriptor(f._x, num_point)
+        out[i].grid = a.descriptor()
+    return out
+
+
+def g2p2g(p2g_functor, points, num_point, spline, halo, array, g2p_functor, g2p_array=None,
+          bins=None):
+    """g2p2g(scalar_p2g, points, num_point, halo, scalar_array, scalar_g2p)
+    (Cabana_Grid_InterpolationPerformance.cpp:130-174) for any functors and
+    spline order: gather the G2P arrays (``g2p_array``, else ``array``),
+    G2P of ``g2p_functor`` (each adds into its particle field), then P2G of
+    ``p2g_functor``, reading the particle fields as the G2P left them, into
+    ``array``, then fold.  Every G2P read sees the arrays as gathered at the
+    start of the call, as the reference's Serial and OpenMP scatter views
+    do.  Functors and arrays may be lists; without a cell binning it is one
+    pass over the particles."""
+    pf = p2g_functor if isinstance(p2g_functor, (list, tuple)) else [p2g_functor]
+    pa = array if isinstance(array, (list, tuple)) else [array] * len(pf)
+    gf = g2p_functor if isinstance(g2p_functor, (list, tuple)) else [g2p_functor]
+    ga = g2p_array if g2p_array is not None else array
+    ga = ga if isinstance(ga, (list, tuple)) else [ga] * len(gf)
+    if len(pa) != len(pf) or len(ga) != len(gf):
+        raise ValueError("one array per functor")
+    m = pa[0].layout().localGrid().mesh_descriptor()
+    pts = _points_descriptor(points, num_point)
+    gt = _terms("g2p", gf, ga, num_point)
+    pt = _terms("p2g", pf, pa, num_point)
+    b = _bins_descriptor(bins)
+    ctx = N.context()
+    N.check(N.load().cabana_amd_g2p2g(ctx, ctypes.byref(m), spline.order, ctypes.byref(pts),
+                                      len(gf), gt, len(pf), pt, _halo_width(halo),
+                                      ctypes.byref(b) if b is not None else None,
+                                      N.current_stream()), ctx)
+
+
 def p2g_adjoint(functor, points, num_point, spline, halo, grid_bar, values_bar,
                 points_bar=None, bins=None):
     """Adjoint of p2g: values_bar (+=) and points_bar (+=) from the cotangent
@@ -786,3 +829,39 @@ class BovWriter:
 
 class Experimental:
     BovWriter = BovWriter
+
+
+# ---------------------------------------------------------------------------
+# Kernel timing
+# ---------------------------------------------------------------------------
+
+
+def profile_enable(enable=True):
+    """Turns the library's per-launch HIP-event timing on or off (the
+    counters are kept; profile_reset clears them)."""
+    ctx = N.context()
+    N.check(N.load().cabana_amd_profile_enable(ctx, 1 if enable else 0), ctx)
+
+
+def profile_reset():
+    ctx = N.context()
+    N.check(N.load().cabana_amd_profile_reset(ctx), ctx)
+
+
+def profile_report():
+    """{kernel family: {"launches", "total_ms", "avg_ms"}} of the launches
+    timed so far (waits for their events)."""
+    lib = N.load()
+    ctx = N.context()
+    cnt = ctypes.c_int32()
+    N.check(lib.cabana_amd_profile_count(ctx, ctypes.byref(cnt)), ctx)
+    out = {}
+    for i in range(cnt.value):
+        name = ctypes.create_string_buffer(64)
+        launches = ctypes.c_int64()
+        ms = ctypes.c_double()
+        N.check(lib.cabana_amd_profile_entry(ctx, i, name, 64, ctypes.byref(launches),
+                                             ctypes.byref(ms)), ctx)
+        out[name.value.decode()] = {"launches": launches.value, "total_ms": ms.value,
+                                    "avg_ms": ms.value / max(launches.value, 1)}
+    return out

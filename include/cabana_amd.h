@@ -209,9 +209,12 @@ cabana_amd_status cabana_amd_check_domain( cabana_amd_context* ctx,
 /* ----------------------------------------------------------------------- */
 
 /* p2g(functor, points, num_point, Spline<order>, halo, array)
- * (Cabana_Grid_Interpolation.hpp:1217-1313) for 1 to 16 functors (the MPM
- * set ScalarValue + VectorValue + TensorDivergence runs in one pass over the
- * particles, other sets in one pass per functor).  Each term ADDS its contributions into its grid
+ * (Cabana_Grid_Interpolation.hpp:1217-1313) for 1 to 16 functors.  With
+ * cell tiles (bins given, orders 1 to 3) ScalarValue + VectorValue (+
+ * TensorDivergence), in any order, run as one fused pass and any other
+ * term as a pass of its own; without them every set of terms is one pass
+ * over the particles (the reference benchmark's fused_p2g,
+ * benchmark/grid/Cabana_Grid_InterpolationPerformance.cpp:25-77).  Each term ADDS its contributions into its grid
  * (owned and ghost nodes), then every distinct grid is folded once: the
  * ghost nodes within halo_width are summed into their periodic owners
  * (Halo::scatter with ScatterReduce::Sum).  Ghost nodes keep their partial
@@ -258,7 +261,10 @@ cabana_amd_status cabana_amd_p2g_no_alloc( cabana_amd_context* ctx,
  * (Cabana_Grid_Interpolation.hpp:831-927) for 1 to 16 functors.  Every
  * distinct grid first gets its ghosts filled from their periodic owners
  * (Halo::gather), then each term ADDS multiplier * result into its particle
- * field. */
+ * field.  With cell tiles VectorValue + VectorGradient from one grid, in
+ * either order, run as one fused pass and other terms one pass each;
+ * without them every set is one pass (the reference benchmark's fused_g2p,
+ * Cabana_Grid_InterpolationPerformance.cpp:81-127). */
 cabana_amd_status cabana_amd_g2p( cabana_amd_context* ctx,
                                   const cabana_amd_mesh* mesh,
                                   int32_t spline_order,
@@ -268,6 +274,25 @@ cabana_amd_status cabana_amd_g2p( cabana_amd_context* ctx,
                                   int32_t halo_width,
                                   const cabana_amd_binning* bins,
                                   cabana_amd_stream stream );
+
+/* g2p2g(scalar_p2g, points, num_point, halo, scalar_array, scalar_g2p)
+ * (benchmark/grid/Cabana_Grid_InterpolationPerformance.cpp:130-174) for
+ * any terms and spline order: Halo::gather of every distinct G2P grid, then
+ * for every particle the G2P terms in order (each adds into its particle
+ * field) followed by the P2G terms (each reads its particle field as the G2P
+ * terms left it and adds into its grid), then Halo::scatter(Sum) of every
+ * distinct P2G grid.  Every G2P read sees the grids as gathered at the
+ * start of the call, as the reference's duplicated (Serial, OpenMP)
+ * scatter views do; a grid that is both read and added into is read from a
+ * device copy.  Without cell tiles (bins NULL, or orders 0 and 4 to 6) this
+ * is one pass over the particles; with them, the G2P passes then the P2G
+ * passes of cabana_amd_g2p and cabana_amd_p2g. */
+cabana_amd_status cabana_amd_g2p2g(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t spline_order,
+    const cabana_amd_particle_field* points, int32_t num_g2p_terms,
+    const cabana_amd_g2p_term* g2p_terms, int32_t num_p2g_terms,
+    const cabana_amd_p2g_term* p2g_terms, int32_t halo_width,
+    const cabana_amd_binning* bins, cabana_amd_stream stream );
 
 /* ----------------------------------------------------------------------- */
 /* Adjoints (no reference counterpart: Cabana has no differentiation)       */
