@@ -79,7 +79,7 @@ def algorithmic_bytes(kernel, n, s, nodes):
 
 
 class Workload:
-    def __init__(self, torch, ca, dtype, cells, ppc, order, seed=20260102):
+    def __init__(self, torch, ca, dtype, cells, ppc, order, seed=20260102, sort=True):
         self.torch = torch
         self.ca = ca
         self.dtype = dtype
@@ -95,7 +95,9 @@ class Workload:
         pos = torch.rand(n, 3, dtype=torch.float64, device=dev, generator=g) * L
         # Cell-sort once (LinkedCellList over the mesh cells + permute).
         self.lcl = ca.createLinkedCellList(pos, [1.0] * 3, [0.0] * 3, [L] * 3)
-        ca.permute(self.lcl, pos)
+        if sort:
+            ca.permute(self.lcl, pos)
+        self.bins = self.lcl if sort else None
         self.pos = pos
         mass = (torch.rand(n, dtype=torch.float64, device=dev, generator=g) + 0.5)
         vel = torch.randn(n, 3, dtype=torch.float64, device=dev, generator=g)
@@ -146,15 +148,15 @@ class Workload:
         for a in (self.gmass, self.gmom, self.gstress, self.gbar_u):
             a.view().zero_()
         ca.p2g(self.p2g_f, self.pos, self.n, self.sp, self.halo,
-               [self.gmass, self.gmom, self.gstress], bins=self.lcl)
+               [self.gmass, self.gmom, self.gstress], bins=self.bins)
         ca.g2p([self.u, self.u], self.halo, self.pos, self.n, self.sp, self.g2p_f,
-               bins=self.lcl)
+               bins=self.bins)
         ca.g2p_adjoint([self.u, self.u], self.halo, self.pos, self.n, self.sp, self.g2p_adj_f,
                        [self.ubar_v, self.ubar_L], [self.gbar_u, self.gbar_u], self.xbar,
-                       bins=self.lcl)
+                       bins=self.bins)
         ca.p2g_adjoint(self.p2g_f, self.pos, self.n, self.sp, self.halo,
                        [self.gbar_m, self.gbar_q, self.gbar_s],
-                       [self.mbar, self.qbar, self.sbar], self.xbar, bins=self.lcl)
+                       [self.mbar, self.qbar, self.sbar], self.xbar, bins=self.bins)
 
     def outputs_of_one_step(self):
         """Every output of one step from zeroed accumulators, as FP64 device
@@ -181,6 +183,29 @@ class Workload:
         positions and the 13 particle attributes."""
         self.lcl.build(self.pos, 0, self.n)
         self.ca.permute(self.lcl, self.pos, self.m, self.q, self.sig)
+
+
+def unbinned_figure(torch, ca, cells, ppc, order, steps=3, warmup=1):
+    """The same FP32-mixed step through the drop-in call shape: no binning
+    passed.  By default each call bins the particles by cell itself and runs
+    the cell-tiled kernels through the sorted permutation (particles in
+    random order, and already in cell order); with that switched off every
+    transfer runs the per-particle kernels with global atomics."""
+    out = {}
+    for label, sort, auto in (("random_order", False, True), ("cell_sorted_order", True, True),
+                              ("random_order_per_particle", False, False),
+                              ("cell_sorted_order_per_particle", True, False)):
+        w = Workload(torch, ca, torch.float32, cells, ppc, order, sort=sort)
+        w.bins = None
+        ca.set_auto_binning(auto)
+        try:
+            ms = time_steps(torch, w.step, steps, warmup)
+        finally:
+            ca.set_auto_binning(True)
+        out[label] = {"ms_per_step": ms, "particles_per_s": w.n / (ms * 1e-3)}
+        del w
+        torch.cuda.empty_cache()
+    return out
 
 
 def field_errors(torch, got, ref):
@@ -587,6 +612,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_extra and "fp32" in res and args.cells == 128:
         extra["differentiation"] = differentiation_figure(torch, ca)
+        torch.cuda.empty_cache()
+        extra["unbinned"] = unbinned_figure(torch, ca, args.cells, args.ppc, args.order)
         torch.cuda.empty_cache()
         extra["checkpointed_loop"] = loop_figure(torch, ca)
         torch.cuda.empty_cache()

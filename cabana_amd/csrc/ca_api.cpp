@@ -70,7 +70,7 @@ struct cabana_amd_context : public Launcher
     unsigned long long* bad = nullptr;     // device counter (binning)
 
     // Reusable scratch.
-    static constexpr int NUM_SLOTS = 6;
+    static constexpr int NUM_SLOTS = 7;
     void* scratch[NUM_SLOTS] = {};
     size_t scratch_bytes[NUM_SLOTS] = {};
 
@@ -97,6 +97,13 @@ struct cabana_amd_context : public Launcher
     static constexpr int PART_TABLES = 8;
     PartTable tables[PART_TABLES];
     uint64_t table_clock = 0;
+
+    // The cell binning the library builds itself for calls that pass none
+    // (see auto_bins): counts, offsets and the particle of each binned
+    // position, in SLOT_BINS.
+    bool auto_binning = true;
+    cabana_amd_binning auto_bins = {};
+    const int64_t* auto_perm = nullptr;
 
     // Kernel timing.
     bool profiling = false;
@@ -193,7 +200,8 @@ enum Slot
     SLOT_SCAN = 2,
     SLOT_PERMUTE = 3,
     SLOT_TILES = 4,
-    SLOT_SNAPSHOT = 5 // copy of a grid that one pass both reads and adds into
+    SLOT_SNAPSHOT = 5, // copy of a grid that one pass both reads and adds into
+    SLOT_BINS = 6      // the library's own cell binning of an unbinned call
 };
 
 cabana_amd_status fail( cabana_amd_context* ctx, cabana_amd_status st,
@@ -952,6 +960,7 @@ cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
                 "scatter (atomic)" );
         return CABANA_AMD_OK;
     }
+    t.perm = bins == &ctx->auto_bins ? ctx->auto_perm : nullptr;
     if ( bins->begin > 0 )
     {
         a.begin = 0;
@@ -1001,6 +1010,7 @@ cabana_amd_status gather_pass( cabana_amd_context* ctx, int order, bool fp64,
         CA_HIP( ctx, run_gather( ctx, order, fp64, plan, a, s ), "gather" );
         return CABANA_AMD_OK;
     }
+    t.perm = bins == &ctx->auto_bins ? ctx->auto_perm : nullptr;
     if ( bins->begin > 0 )
     {
         a.begin = 0;
@@ -1047,6 +1057,66 @@ cabana_amd_status validate_bins( cabana_amd_context* ctx,
     if ( bins->begin < 0 || bins->end < bins->begin || bins->end > n )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                      "binning range outside the particle range" );
+    return CABANA_AMD_OK;
+}
+
+// The drop-in call shape passes no binning.  Then, for the orders with
+// cell-tiled kernels, the library bins the particles itself: owned-cell
+// keys, the stable radix sort of cabana_amd_linked_cell_build, counts and
+// offsets, kept in SLOT_BINS with the sorted order as a permutation that
+// the tiled kernels read particles through (the particles stay where they
+// are).  `bins` is left as it is when the call passed one, when the
+// context has the feature off, or when it cannot apply.
+cabana_amd_status auto_bins( cabana_amd_context* ctx, const MeshDev& m, int order,
+                             const PFieldDev& pts, int64_t n,
+                             const cabana_amd_binning*& bins, hipStream_t s )
+{
+    if ( bins || !ctx->auto_binning || order < 1 || order > 3 || n <= 0 )
+        return CABANA_AMD_OK;
+    const int64_t ncell = (int64_t)m.num_cell[0] * m.num_cell[1] * m.num_cell[2];
+    if ( ncell >= INT32_MAX / 2 )
+        return CABANA_AMD_OK;
+    int key_bits = 0;
+    while ( ( ( ncell - 1 ) >> key_bits ) != 0 )
+        ++key_bits;
+    int digit_bits;
+    lcl_sort_passes( key_bits, &digit_bits );
+    const int64_t nhist = ( (int64_t)1 << digit_bits ) * lcl_sort_tiles( n );
+    const int64_t nblock = ( nhist + 1023 ) / 1024 + 1;
+    const int64_t nkey = n + 64;
+    CA_HIP( ctx, ctx->ensure( SLOT_CELL_OF, sizeof( int32_t ) * 4 * nkey ), "scratch" );
+    CA_HIP( ctx,
+            ctx->ensure( SLOT_CURSOR, ( sizeof( int64_t ) + sizeof( int32_t ) ) * ( nhist + 64 ) ),
+            "scratch" );
+    CA_HIP( ctx, ctx->ensure( SLOT_SCAN, sizeof( int64_t ) * nblock ), "scratch" );
+    const size_t o_off = a16( sizeof( int32_t ) * (size_t)ncell );
+    const size_t o_perm = a16( o_off + sizeof( int64_t ) * (size_t)ncell );
+    CA_HIP( ctx, ctx->ensure( SLOT_BINS, o_perm + sizeof( int64_t ) * (size_t)n ), "scratch" );
+    char* base = static_cast<char*>( ctx->scratch[SLOT_BINS] );
+    int32_t* counts = reinterpret_cast<int32_t*>( base );
+    int64_t* offsets = reinterpret_cast<int64_t*>( base + o_off );
+    int64_t* perm = reinterpret_cast<int64_t*>( base + o_perm );
+    int32_t* keys = static_cast<int32_t*>( ctx->scratch[SLOT_CELL_OF] );
+    int64_t* digit_base = static_cast<int64_t*>( ctx->scratch[SLOT_CURSOR] );
+    int32_t* hist = reinterpret_cast<int32_t*>( digit_base + nhist + 64 );
+    int64_t* sums = static_cast<int64_t*>( ctx->scratch[SLOT_SCAN] );
+    CA_HIP( ctx, launch_auto_bin_keys( *ctx, pts, n, m, keys, s ), "bin keys" );
+    int32_t* sorted = keys;
+    CA_HIP( ctx,
+            launch_lcl_sort( *ctx, n, key_bits, keys, keys + nkey, keys + 2 * nkey,
+                             keys + 3 * nkey, hist, digit_base, sums, nblock, perm, 0, &sorted,
+                             s ),
+            "bin sort" );
+    CA_HIP( ctx, launch_lcl_bounds( *ctx, sorted, n, ncell, counts, offsets, s ), "bin bounds" );
+    ctx->auto_bins = {};
+    ctx->auto_bins.counts = counts;
+    ctx->auto_bins.offsets = offsets;
+    ctx->auto_bins.begin = 0;
+    ctx->auto_bins.end = n;
+    for ( int d = 0; d < 3; ++d )
+        ctx->auto_bins.num_bin[d] = m.num_cell[d];
+    ctx->auto_perm = perm;
+    bins = &ctx->auto_bins;
     return CABANA_AMD_OK;
 }
 
@@ -1206,6 +1276,8 @@ cabana_amd_status p2g_impl( cabana_amd_context* ctx, const cabana_amd_mesh* mesh
                 return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                              "terms share grid memory with different layouts" );
     hipStream_t s = static_cast<hipStream_t>( stream );
+    if ( !fixed )
+        CA_CHECK( auto_bins( ctx, m, order, a.points, n, bins, s ) );
     if ( n > 0 )
     {
         // The fused plans, whatever the order of the terms: (ScalarValue,
@@ -1430,6 +1502,7 @@ extern "C" cabana_amd_status cabana_amd_g2p(
                                "grid", T[i].grid ) );
     }
     hipStream_t s = static_cast<hipStream_t>( stream );
+    CA_CHECK( auto_bins( ctx, m, order, a.points, n, bins, s ) );
     for ( int i = 0; i < num_terms; ++i )
     {
         bool seen = false;
@@ -1478,6 +1551,17 @@ extern "C" cabana_amd_status cabana_amd_g2p2g(
          num_p2g_terms < 1 || num_p2g_terms > MAX_CALL_TERMS || !p2g_terms )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                      "need points, 1..16 G2P terms and 1..16 P2G terms" );
+    if ( !bins && ctx->auto_binning && order >= 1 && order <= 3 )
+    {
+        // The library's own binning, built once for both halves.
+        MeshDev mb;
+        CA_CHECK( make_mesh( ctx, mesh, mb ) );
+        PFieldDev pb;
+        CA_CHECK( make_pfield( ctx, *points, 3, CABANA_AMD_FLOAT64, points->num_particle, false,
+                               "points", pb ) );
+        CA_CHECK( auto_bins( ctx, mb, order, pb, points->num_particle, bins,
+                             static_cast<hipStream_t>( stream ) ) );
+    }
     if ( tiled_call( bins, order ) )
     {
         // Cell-tiled: the G2P passes, then the P2G passes.  Every read of a
@@ -1658,6 +1742,7 @@ extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
                                false, "grid_bar", T[i].grid ) );
     }
     hipStream_t s = static_cast<hipStream_t>( stream );
+    CA_CHECK( auto_bins( ctx, m, order, a.points, n, bins, s ) );
     for ( int i = 0; i < num_terms; ++i )
     {
         bool seen = false;
@@ -1749,6 +1834,7 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
                          "term has neither grid_bar nor points_bar" );
     }
     hipStream_t s = static_cast<hipStream_t>( stream );
+    CA_CHECK( auto_bins( ctx, m, order, a.points, n, bins, s ) );
     // Forward ghost fill of the grids read for the position cotangent.
     if ( xbar )
         for ( int i = 0; i < num_terms; ++i )
@@ -2341,6 +2427,15 @@ extern "C" cabana_amd_status cabana_amd_particle_migrate(
 // ---------------------------------------------------------------------------
 // Profiling
 // ---------------------------------------------------------------------------
+
+extern "C" cabana_amd_status cabana_amd_set_auto_binning( cabana_amd_context* ctx,
+                                                          int32_t enable )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    ctx->auto_binning = enable != 0;
+    return CABANA_AMD_OK;
+}
 
 extern "C" cabana_amd_status cabana_amd_profile_enable( cabana_amd_context* ctx,
                                                         int32_t enable )

@@ -765,6 +765,30 @@ __global__ __launch_bounds__( BLOCK ) void compact_index_kernel( const int32_t* 
         idx[rank[p]] = p;
 }
 
+// Cell of a particle for the library's own binning of a call that passes
+// none: the floor of its logical cell coordinate in the owned cells,
+// clamped into them (a particle outside them, or with a NaN coordinate,
+// still gets a cell; the tiled kernels interpolate a particle whose stencil
+// leaves its brick through global memory).  Key = cardinal cell index.
+__global__ __launch_bounds__( BLOCK ) void auto_bin_key_kernel( PFieldDev pos, int64_t n,
+                                                                MeshDev m, int32_t* keys )
+{
+#pragma clang fp contract( off )
+    const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( p >= n )
+        return;
+    const uint32_t b = pos.base( p );
+    int c[3];
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        const double x = *pos.ptr<const double>( b, d );
+        const double f = floor( ( x - m.low[d] ) * m.rdx[d] ) - (double)m.halo;
+        c[d] = f >= 0.0 ? ( f < (double)m.num_cell[d] ? (int)f : m.num_cell[d] - 1 ) : 0;
+    }
+    keys[p] = ( c[0] * m.num_cell[1] + c[1] ) * m.num_cell[2] + c[2];
+}
+
 // ---------------------------------------------------------------------------
 // BOV output
 // ---------------------------------------------------------------------------
@@ -1265,6 +1289,18 @@ hipError_t launch_bov_pack( Launcher& L, bool fp64, const GFieldDev& g, int lo, 
         hipLaunchKernelGGL( bov_pack_kernel<float>, blocks_for( total ), dim3( BLOCK ), 0, s, g,
                             lo, n[0], n[1], n[2], static_cast<float*>( out ) );
     L.after( "bov_pack", s );
+    return hipGetLastError();
+}
+
+hipError_t launch_auto_bin_keys( Launcher& L, const PFieldDev& pos, int64_t n, const MeshDev& m,
+                                 int32_t* keys, hipStream_t s )
+{
+    if ( n <= 0 )
+        return hipSuccess;
+    L.before( "auto_bin_keys", s );
+    hipLaunchKernelGGL( auto_bin_key_kernel, blocks_for( n ), dim3( BLOCK ), 0, s, pos, n, m,
+                        keys );
+    L.after( "auto_bin_keys", s );
     return hipGetLastError();
 }
 

@@ -333,6 +333,14 @@ struct TileArgs
     int32_t max_tasks;           // scatter: task-table capacity
     int32_t grid;                // workgroups launched (a multiple of 8)
     int64_t bin_begin;           // particle index of cell offset 0
+    // Particle at each binned position q (perm[q - bin_begin]) when the
+    // binning does not reorder the particles themselves; nullptr: the
+    // particle at q is q.
+    const int64_t* perm;
+    __device__ __forceinline__ int64_t particle( int64_t q ) const
+    {
+        return perm ? perm[q - bin_begin] : q;
+    }
 };
 
 // LDS record of one particle in the cell-sorted scatter: 3 fractional
@@ -505,6 +513,10 @@ hipError_t launch_migrate_count( Launcher& L, const PFieldDev& pos, int64_t n,
 hipError_t launch_migrate_shift( Launcher& L, const PFieldDev& pos, int64_t n,
                                  const MigrateBoundsDev& b, int32_t* keep,
                                  unsigned long long* dropped, hipStream_t s );
+// Owned-cell keys of particles [0, n) for the library's own binning
+// (clamped into the owned cells).
+hipError_t launch_auto_bin_keys( Launcher& L, const PFieldDev& pos, int64_t n, const MeshDev& m,
+                                 int32_t* keys, hipStream_t s );
 // Block [lo, lo + n) of every dimension of a grid array, packed x fastest
 // after the dofs (the BOV data order) into out.
 hipError_t launch_bov_pack( Launcher& L, bool fp64, const GFieldDev& g, int lo, const int n[3],

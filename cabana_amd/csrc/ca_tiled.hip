@@ -173,7 +173,7 @@ __device__ __forceinline__ int64_t run_particle( const TileArgs& ta,
         else
             hi = mid - 1;
     }
-    return ta.bin_begin + run_start[lo] + ( v - run_pre[lo] );
+    return ta.particle( ta.bin_begin + run_start[lo] + ( v - run_pre[lo] ) );
 }
 
 // Run containing flattened brick position v (binary search).
@@ -303,7 +303,7 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
         if ( v < pend )
         {
             r = find_run( run_pre, nruns, v );
-            p = ta.bin_begin + run_start[r] + ( v - run_pre[r] );
+            p = ta.particle( ta.bin_begin + run_start[r] + ( v - run_pre[r] ) );
             load_position<CONTIG>( a.points, p, px );
             gather_load<Real, MODE, CONTIG, FORMS...>( a, p, in );
         }
@@ -317,7 +317,7 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
             {
                 while ( run_pre[r + 1] <= vn )
                     ++r;
-                pn = ta.bin_begin + run_start[r] + ( vn - run_pre[r] );
+                pn = ta.particle( ta.bin_begin + run_start[r] + ( vn - run_pre[r] ) );
                 load_position<CONTIG>( a.points, pn, pxn );
                 gather_load<Real, MODE, CONTIG, FORMS...>( a, pn, inn );
             }
@@ -813,7 +813,8 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
         pf.p[mi] = -1;
         if ( i >= nbatch )
             continue;
-        const int64_t p = ta.bin_begin + run_start[run[mi]] + ( pbeg + i - run_pre[run[mi]] );
+        const int64_t p =
+            ta.particle( ta.bin_begin + run_start[run[mi]] + ( pbeg + i - run_pre[run[mi]] ) );
         pf.p[mi] = (int)p;
         load_position<CONTIG>( a.points, p, pf.x[mi] );
         static_for<FL::N>( [&]( auto I ) {

@@ -865,3 +865,11 @@ def profile_report():
         out[name.value.decode()] = {"launches": launches.value, "total_ms": ms.value,
                                     "avg_ms": ms.value / max(launches.value, 1)}
     return out
+
+
+def set_auto_binning(enable=True):
+    """Whether calls without a binning bin the particles by cell themselves
+    and run the cell-tiled kernels (on by default) or run the per-particle
+    kernels with global atomics."""
+    ctx = N.context()
+    N.check(N.load().cabana_amd_set_auto_binning(ctx, 1 if enable else 0), ctx)

@@ -453,6 +453,21 @@ cabana_amd_status cabana_amd_particle_migrate(
     int32_t* migrated, cabana_amd_stream stream );
 
 /* ----------------------------------------------------------------------- */
+/* The library's own binning                                                */
+/* ----------------------------------------------------------------------- */
+
+/* Calls that pass no binning (bins NULL, the drop-in shape of Cabana's p2g
+ * and g2p) with spline orders 1 to 3 bin the particles by owned cell
+ * themselves on the call's stream (cell keys, a stable radix sort, counts
+ * and offsets, in context scratch) and run the cell-tiled kernels, reading
+ * the particles in cell order through the sorted permutation; the
+ * particles are not moved.  Results are those of the per-particle path up
+ * to the order of floating-point additions.  enable = 0 turns this off for
+ * the context (every unbinned call then runs the per-particle kernels with
+ * global atomics); it is on by default.  No reference counterpart. */
+cabana_amd_status cabana_amd_set_auto_binning( cabana_amd_context* ctx, int32_t enable );
+
+/* ----------------------------------------------------------------------- */
 /* Kernel timing (HIP events around every launch of this context)           */
 /* ----------------------------------------------------------------------- */
 

@@ -2,6 +2,8 @@
 spline orders 1-3, FP64 (rel-L2 <= 1e-12) and FP32-mixed (rel-L2 <= 1e-5),
 particle layouts (AoS, SoA, AoSoA), grid layouts (LayoutRight/Left), the
 fused MPM term sets and the cell-sorted tiled scatter."""
+import contextlib
+
 import numpy as np
 import pytest
 import torch
@@ -135,13 +137,28 @@ def sort_case(ca, c):
     return lcl, perm
 
 
-@pytest.mark.parametrize("sorted_bins", [False, True])
+@contextlib.contextmanager
+def per_particle_path(ca, on):
+    """Unbinned calls without the library's own binning (the per-particle
+    kernels with global atomics) while `on`."""
+    if on:
+        ca.set_auto_binning(False)
+    try:
+        yield
+    finally:
+        if on:
+            ca.set_auto_binning(True)
+
+
+# sorted_bins: True = the caller's LinkedCellList; False = none (the
+# library bins the particles itself); "atomic" = none, per-particle path.
+@pytest.mark.parametrize("sorted_bins", [False, True, "atomic"])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("order", [1, 2, 3])
 @pytest.mark.parametrize("op", range(5))
 def test_p2g_matches_oracle(ca, oracle, op, order, dtype, sorted_bins):
     c = Case(ca, oracle, seed=op * 7 + order)
-    bins = sort_case(ca, c)[0] if sorted_bins else None
+    bins = sort_case(ca, c)[0] if sorted_bins is True else None
     nc, nd = P2G_SHAPE[op]
     vals = c.rng.normal(size=(c.n, nc))
     grid = c.array(nd, dtype)
@@ -150,7 +167,8 @@ def test_p2g_matches_oracle(ca, oracle, op, order, dtype, sorted_bins):
     grid.view().copy_(torch.tensor(init, dtype=dtype))
     f = getattr(ca, P2G_FACTORY[op])(shaped(torch.tensor(vals, dtype=dtype, device="cuda"),
                                             nc), -0.6)
-    ca.p2g(f, c.pts, c.n, ca.Spline(order), c.halo(grid), grid, bins=bins)
+    with per_particle_path(ca, sorted_bins == "atomic"):
+        ca.p2g(f, c.pts, c.n, ca.Spline(order), c.halo(grid), grid, bins=bins)
     ref = init.copy()
     oracle.p2g(c.mesh, order, op, -0.6, c.pts_np, vals, ref)
     own = c.mesh.owned_slices()
@@ -159,13 +177,13 @@ def test_p2g_matches_oracle(ca, oracle, op, order, dtype, sorted_bins):
     assert ca.check_domain() == 0
 
 
-@pytest.mark.parametrize("sorted_bins", [False, True])
+@pytest.mark.parametrize("sorted_bins", [False, True, "atomic"])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("order", [1, 2, 3])
 @pytest.mark.parametrize("op", range(5))
 def test_g2p_matches_oracle(ca, oracle, op, order, dtype, sorted_bins):
     c = Case(ca, oracle, seed=100 + op * 7 + order)
-    bins = sort_case(ca, c)[0] if sorted_bins else None
+    bins = sort_case(ca, c)[0] if sorted_bins is True else None
     nc, nd = G2P_SHAPE[op]
     grid = c.array(nd, dtype)
     g = c.rng.normal(size=grid.view().shape)
@@ -173,7 +191,8 @@ def test_g2p_matches_oracle(ca, oracle, op, order, dtype, sorted_bins):
     base = c.rng.normal(size=(c.n, nc))
     out = shaped(torch.tensor(base, dtype=dtype, device="cuda"), nc)
     f = getattr(ca, G2P_FACTORY[op])(out, 1.7)
-    ca.g2p(grid, c.halo(grid), c.pts, c.n, ca.Spline(order), f, bins=bins)
+    with per_particle_path(ca, sorted_bins == "atomic"):
+        ca.g2p(grid, c.halo(grid), c.pts, c.n, ca.Spline(order), f, bins=bins)
     ref = base.copy()
     gref = g.copy()
     oracle.g2p(c.mesh, order, op, 1.7, gref, c.pts_np, ref)
@@ -561,4 +580,55 @@ def test_mass_momentum_pair(ca, oracle, order, dtype, vl):
         oracle.p2g(c.mesh, order, op, mult, pts_np, vals, ref)
         err = rel_l2(arr.view().cpu().numpy()[own], ref[own])
         assert err <= TOL[dtype], (op, err)
+    assert ca.check_domain() == 0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_library_binning_with_particles_in_the_halo(ca, oracle, order, dtype):
+    """Unbinned calls bin by owned cell themselves; particles in the ghost
+    layer (their keys clamp into a boundary cell) and clustered particles
+    still match the oracle, for the MPM P2G set and the G2P pair, and the
+    library's own binning is what runs (its kernels appear in the profile)."""
+    c = Case(ca, oracle, num_cell=(12, 10, 9), n=6000, halo=2, seed=31 + order)
+    lo = np.array(c.mesh.m.global_low)
+    hi = np.array(c.mesh.m.global_high)
+    cell = 0.25
+    pts = c.pts_np.copy()
+    k = c.n // 10
+    pts[:k] = lo + c.rng.uniform(-0.9, 0.0, size=(k, 3)) * cell
+    pts[k:2 * k] = hi + c.rng.uniform(0.0, 0.9, size=(k, 3)) * cell
+    pts[2 * k:4 * k] = lo + (hi - lo) * 0.5 + c.rng.normal(size=(2 * k, 3)) * cell
+    c.pts_np = pts
+    c.pts = torch.tensor(pts, device="cuda")
+    m, q, sig = mpm_fields(c, dtype)
+    t = lambda a: torch.tensor(a, dtype=dtype, device="cuda")  # noqa: E731
+    grids = [c.array(nd, dtype) for nd in (1, 3, 3)]
+    ca.profile_reset()
+    ca.profile_enable(True)
+    ca.p2g([ca.createScalarValueP2G(t(m[:, 0]), 1.0), ca.createVectorValueP2G(t(q), 1.0),
+            ca.createTensorDivergenceP2G(t(sig.reshape(-1, 3, 3)), -0.25)], c.pts, c.n,
+           ca.Spline(order), c.halo(grids[0]), grids)
+    u = c.array(3, dtype)
+    un = c.rng.normal(size=u.view().shape)
+    u.view().copy_(torch.tensor(un, dtype=dtype))
+    vel = torch.zeros(c.n, 3, dtype=dtype, device="cuda")
+    grad = torch.zeros(c.n, 3, 3, dtype=dtype, device="cuda")
+    ca.g2p([u, u], c.halo(u), c.pts, c.n, ca.Spline(order),
+           [ca.createVectorValueG2P(vel, 1.0), ca.createVectorGradientG2P(grad, 1.0)])
+    prof = ca.profile_report()
+    ca.profile_enable(False)
+    assert "auto_bin_keys" in prof and "p2g_mpm_cells" in prof and "g2p_mpm_tiled" in prof
+    own = c.mesh.owned_slices()
+    for arr, op, vals, mult, nd in ((grids[0], 0, m, 1.0, 1), (grids[1], 1, q, 1.0, 3),
+                                    (grids[2], 4, sig, -0.25, 3)):
+        ref = c.mesh.grid(nd)
+        oracle.p2g(c.mesh, order, op, mult, c.pts_np, vals, ref)
+        assert rel_l2(arr.view().cpu().numpy()[own], ref[own]) <= TOL[dtype], op
+    vref = np.zeros((c.n, 3))
+    gref = np.zeros((c.n, 9))
+    oracle.g2p(c.mesh, order, 1, 1.0, un.copy(), c.pts_np, vref)
+    oracle.g2p(c.mesh, order, 3, 1.0, un.copy(), c.pts_np, gref)
+    assert rel_l2(vel.cpu().numpy(), vref) <= TOL[dtype]
+    assert rel_l2(grad.cpu().numpy().reshape(-1, 9), gref) <= TOL[dtype]
     assert ca.check_domain() == 0
