@@ -3,7 +3,8 @@ p2g / g2p operator semantics.  The compute runs in hand-written HIP kernels
 (libcabana_amd.so); this package is the host-side mirror of the reference
 interface over the C ABI."""
 from .grid import (  # noqa: F401
-    AoSoASlice, Array, ArrayLayout, ArrayOp, BovWriter, Cell, Experimental, Ghost, GlobalGrid,
+    AoSoASlice, Array, ArrayLayout, ArrayOp, BovWriter, Cell, Dim, Edge, Experimental, Face, Ghost,
+    GlobalGrid,
     GlobalMesh, Halo,
     IndexSpace, Local, LocalGrid, LocalMesh, Node, NodeHaloPattern, Own, ScatterReduce,
     Spline, check_domain, createArray, createArrayLayout, createGlobalGrid, createHalo,

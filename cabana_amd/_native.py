@@ -46,6 +46,7 @@ class Mesh(ctypes.Structure):
         ("cell_size", c_double * 3),
         ("halo_cell_width", c_int32),
         ("periodic", c_int32 * 3),
+        ("entity", c_int32),
     ]
 
 

@@ -264,6 +264,8 @@ cabana_amd_status make_mesh( cabana_amd_context* ctx,
     if ( mesh->halo_cell_width < 0 )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                      "negative halo cell width" );
+    if ( mesh->entity < CABANA_AMD_NODE || mesh->entity > CABANA_AMD_EDGE_K )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "unknown mesh entity" );
     for ( int d = 0; d < 3; ++d )
     {
         const double cell = mesh->cell_size[d];
@@ -279,13 +281,24 @@ cabana_amd_status make_mesh( cabana_amd_context* ctx,
         if ( std::fabs( n * cell - extent ) > 100.0 * 2.220446049250313e-16 )
             return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                          "Extent not evenly divisible by uniform cell size" );
+        // Entity per dimension (Cabana_Grid_LocalMesh.hpp:176-237): Cell at
+        // cell centres, Face<D> at nodes only in D, Edge<D> at centres only
+        // in D, Node at nodes.
+        const int e = mesh->entity;
+        const int stag = e == CABANA_AMD_NODE   ? 0
+                         : e == CABANA_AMD_CELL ? 1
+                         : e <= CABANA_AMD_FACE_K ? ( d == e - CABANA_AMD_FACE_I ? 0 : 1 )
+                                                  : ( d == e - CABANA_AMD_EDGE_I ? 1 : 0 );
+        m.stag[d] = stag;
         m.num_cell[d] = n;
-        m.num_node[d] = n + 2 * mesh->halo_cell_width + 1;
+        m.num_node[d] = n + 2 * mesh->halo_cell_width + 1 - stag;
         m.periodic[d] = mesh->periodic[d] ? 1 : 0;
+        // evaluateSpline (Cabana_Grid_Splines.hpp:1283-1318): low corner =
+        // coordinates of entity 0, dx = coordinates of entity 1 minus it.
         const double own_low = mesh->low_corner[d] + cell * 0;
         const double ghost_low = own_low - mesh->halo_cell_width * cell;
-        const double x0 = ghost_low + (double)0 * cell;
-        const double x1 = ghost_low + (double)1 * cell;
+        const double x0 = stag ? ghost_low + ( 0.0 + 0.5 ) * cell : ghost_low + (double)0 * cell;
+        const double x1 = stag ? ghost_low + ( 1.0 + 0.5 ) * cell : ghost_low + (double)1 * cell;
         m.low[d] = x0;
         m.rdx[d] = 1.0 / ( x1 - x0 );
     }
@@ -565,11 +578,14 @@ hipError_t run_multi_term( cabana_amd_context* ctx, int order, bool fp64,
     }
 }
 
-// Whether a pass over these particles can run the cell-tiled kernels.
-bool tiled_call( const cabana_amd_binning* bins, int order )
+// Whether a pass over these particles can run the cell-tiled kernels (Node
+// arrays only).
+bool tiled_call( const cabana_amd_binning* bins, int order, bool node_arrays )
 {
-    return bins && bins->counts && bins->offsets && order >= 1 && order <= 3;
+    return bins && bins->counts && bins->offsets && order >= 1 && order <= 3 && node_arrays;
 }
+
+bool node_arrays( const MeshDev& m ) { return !m.stag[0] && !m.stag[1] && !m.stag[2]; }
 
 // Index of the first term of a form, or -1.
 template <class Term>
@@ -741,7 +757,7 @@ bool make_tiles( TileKind kind, const MeshDev& m, int order, bool fp64,
                  const cabana_amd_binning* bins, const GFieldDev& g0,
                  TileArgs& t, size_t& lds )
 {
-    if ( order < 1 || order > 3 )
+    if ( order < 1 || order > 3 || m.stag[0] || m.stag[1] || m.stag[2] )
         return false;
     if ( !bins || !bins->counts || !bins->offsets )
         return false;
@@ -1071,7 +1087,8 @@ cabana_amd_status auto_bins( cabana_amd_context* ctx, const MeshDev& m, int orde
                              const PFieldDev& pts, int64_t n,
                              const cabana_amd_binning*& bins, hipStream_t s )
 {
-    if ( bins || !ctx->auto_binning || order < 1 || order > 3 || n <= 0 )
+    if ( bins || !ctx->auto_binning || order < 1 || order > 3 || n <= 0 || m.stag[0] ||
+         m.stag[1] || m.stag[2] )
         return CABANA_AMD_OK;
     const int64_t ncell = (int64_t)m.num_cell[0] * m.num_cell[1] * m.num_cell[2];
     if ( ncell >= INT32_MAX / 2 )
@@ -1299,7 +1316,7 @@ cabana_amd_status p2g_impl( cabana_amd_context* ctx, const cabana_amd_mesh* mesh
                 taken[fused[k]] = true;
         }
         const int rest = num_terms - nfused;
-        if ( num_terms > 1 && rest > 0 && !tiled_call( bins, order ) )
+        if ( num_terms > 1 && rest > 0 && !tiled_call( bins, order, node_arrays( m ) ) )
         {
             // Untiled: every term in one pass (the spline once per particle).
             MultiTermArgs ma;
@@ -1420,7 +1437,7 @@ cabana_amd_status g2p_passes( cabana_amd_context* ctx, int order, bool fp64, con
     const int nfused = pair[0] >= 0 ? 2 : 0;
     if ( nfused )
         taken[pair[0]] = taken[pair[1]] = true;
-    if ( num_terms > 1 && num_terms > nfused && !tiled_call( bins, order ) )
+    if ( num_terms > 1 && num_terms > nfused && !tiled_call( bins, order, node_arrays( a.mesh ) ) )
     {
         MultiTermArgs ma;
         std::memset( &ma, 0, sizeof( ma ) );
@@ -1562,7 +1579,7 @@ extern "C" cabana_amd_status cabana_amd_g2p2g(
         CA_CHECK( auto_bins( ctx, mb, order, pb, points->num_particle, bins,
                              static_cast<hipStream_t>( stream ) ) );
     }
-    if ( tiled_call( bins, order ) )
+    if ( tiled_call( bins, order, mesh && mesh->entity == CABANA_AMD_NODE ) )
     {
         // Cell-tiled: the G2P passes, then the P2G passes.  Every read of a
         // grid precedes every addition into it, as in the reference's
@@ -2210,18 +2227,18 @@ extern "C" cabana_amd_status cabana_amd_bov_write_time_step(
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "NULL argument" );
     if ( entity != CABANA_AMD_NODE && entity != CABANA_AMD_CELL )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "entity must be NODE or CELL" );
+    if ( !mesh )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "mesh is NULL" );
+    // The array's entity is the call's argument (the writer's centering).
+    cabana_amd_mesh me = *mesh;
+    me.entity = entity;
     MeshDev m;
-    CA_CHECK( make_mesh( ctx, mesh, m ) );
+    CA_CHECK( make_mesh( ctx, &me, m ) );
     bool fp64 = true;
     CA_CHECK( precision_of( ctx, grid->dtype, fp64 ) );
     const bool node = entity == CABANA_AMD_NODE;
-    // A cell array spans num_cell + 2 * halo cells per dimension.
-    MeshDev mg = m;
-    if ( !node )
-        for ( int d = 0; d < 3; ++d )
-            mg.num_node[d] = m.num_cell[d] + 2 * m.halo;
     GFieldDev g;
-    CA_CHECK( make_gfield( ctx, mg, *grid, -1, grid->dtype, false, "grid", g ) );
+    CA_CHECK( make_gfield( ctx, m, *grid, -1, grid->dtype, false, "grid", g ) );
     hipStream_t s = static_cast<hipStream_t>( stream );
     // writeTimeStep (Cabana_Grid_BovWriter.hpp:180-338) on one rank, the
     // last block of every dimension: global extents are the cell counts

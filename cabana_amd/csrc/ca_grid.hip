@@ -14,20 +14,21 @@ inline dim3 blocks_for( int64_t n )
     return dim3( (unsigned)( ( n + BLOCK - 1 ) / BLOCK ) );
 }
 
-// Owned node range of one dimension on one rank
-// (Cabana_Grid_LocalGrid_impl.hpp:430-450): periodic dimensions own N nodes,
-// non-periodic ones also own the closing boundary node.
+// Owned entity range of one dimension on one rank
+// (Cabana_Grid_LocalGrid_impl.hpp:239-280, 430-450, 534-560): N entities,
+// plus the closing boundary node in a non-periodic node-centred dimension.
 __device__ __forceinline__ int owned_hi( const MeshDev& m, int d )
 {
-    return m.halo + m.num_cell[d] + ( m.periodic[d] ? 0 : 1 );
+    return m.halo + m.num_cell[d] + ( ( m.periodic[d] || m.stag[d] ) ? 0 : 1 );
 }
 
 // Classifies node index x of dimension d for a halo of width w.  Returns the
 // owner index (itself when owned), or -1 when the node is a ghost that no
 // message of the 26-direction exchange reaches.  The bands are the shared
-// index spaces of Cabana_Grid_LocalGrid_impl.hpp:470-509: low ghosts
-// [h - w, h) receive from [h + N - w, h + N), high ghosts [h + N, h + N + w + 1)
-// receive from [h, h + w + 1).
+// index spaces of Cabana_Grid_LocalGrid_impl.hpp:370-398, 470-509: low
+// ghosts [h - w, h) receive from [h + N - w, h + N), high ghosts
+// [h + N, h + N + w + p) receive from [h, h + w + p), p = 1 in a
+// node-centred dimension and 0 in a cell-centred one.
 __device__ __forceinline__ int owner_of( const MeshDev& m, int d, int x, int w,
                                          bool& is_ghost )
 {
@@ -40,7 +41,7 @@ __device__ __forceinline__ int owner_of( const MeshDev& m, int d, int x, int w,
         return -1;
     if ( x >= h - w && x < h )
         return x + N;
-    if ( x >= h + N && x < h + N + w + 1 )
+    if ( x >= h + N && x < h + N + w + 1 - m.stag[d] )
         return x - N;
     return -1;
 }
@@ -111,7 +112,7 @@ __global__ __launch_bounds__( BLOCK ) void halo_fold_kernel( MeshDev m,
         nc[d] = 1;
         if ( m.periodic[d] )
         {
-            if ( x[d] < h + w + 1 )
+            if ( x[d] < h + w + 1 - m.stag[d] )
                 cand[d][nc[d]++] = x[d] + N;
             if ( x[d] >= h + N - w )
                 cand[d][nc[d]++] = x[d] - N;

@@ -23,6 +23,7 @@ struct MeshDev
     int32_t num_cell[3];
     int32_t halo;        // halo cell width of the local grid
     int32_t periodic[3];
+    int32_t stag[3];     // 1 where the arrays' entity sits at cell centres
 };
 
 // Particle field addressing (Cabana_Slice.hpp:119-190).  For a power-of-two

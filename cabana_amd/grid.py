@@ -31,11 +31,47 @@ class Local:
 
 
 class Node:
-    pass
+    code = 0
 
 
 class Cell:
-    pass
+    code = 1
+
+
+class Dim:
+    I, J, K = 0, 1, 2
+
+
+class Face:
+    """Face<Dir> tag: faces normal to dimension Dir."""
+
+    def __init__(self, dim):
+        if dim not in (0, 1, 2):
+            raise ValueError("face dimension must be 0, 1 or 2")
+        self.dim = dim
+        self.code = 2 + dim
+
+
+class Edge:
+    """Edge<Dir> tag: edges along dimension Dir."""
+
+    def __init__(self, dim):
+        if dim not in (0, 1, 2):
+            raise ValueError("edge dimension must be 0, 1 or 2")
+        self.dim = dim
+        self.code = 5 + dim
+
+
+def _stag(entity, d):
+    """1 where the entity sits at cell centres in dimension d
+    (Cabana_Grid_LocalMesh.hpp:176-237)."""
+    if isinstance(entity, Node):
+        return 0
+    if isinstance(entity, Cell):
+        return 1
+    if isinstance(entity, Face):
+        return 0 if d == entity.dim else 1
+    return 1 if d == entity.dim else 0
 
 
 class NodeHaloPattern:
@@ -167,19 +203,20 @@ class LocalGrid:
         return self._gg.ownedNumCell(d) + 2 * self._h
 
     def indexSpace(self, decomposition, entity, index_type=None):
+        """Own / Ghost local index space of any entity
+        (Cabana_Grid_LocalGrid_impl.hpp:239-280, 430-470, 534-560): per
+        dimension N entities plus, where the entity is node-centred, the
+        ghosted extra node and, in a non-periodic dimension, the owned
+        closing node."""
         h = self._h
-        if isinstance(entity, Cell):
-            if isinstance(decomposition, Own):
-                return IndexSpace([h] * 3, [h + self._gg.ownedNumCell(d) for d in range(3)])
-            return IndexSpace([0] * 3, [self.totalNumCell(d) for d in range(3)])
+        stag = [_stag(entity, d) for d in range(3)]
         if isinstance(decomposition, Own):
-            # Periodic dimensions resolve the shared node (:430-450).
             return IndexSpace([h] * 3, [
-                h + self._gg.ownedNumCell(d) + (0 if self._gg.isPeriodic(d) else 1)
+                h + self._gg.ownedNumCell(d) + (0 if (stag[d] or self._gg.isPeriodic(d)) else 1)
                 for d in range(3)])
-        return IndexSpace([0] * 3, [self.totalNumCell(d) + 1 for d in range(3)])
+        return IndexSpace([0] * 3, [self.totalNumCell(d) + 1 - stag[d] for d in range(3)])
 
-    def mesh_descriptor(self):
+    def mesh_descriptor(self, entity=None):
         gm = self._gg.globalMesh()
         m = N.Mesh()
         for d in range(3):
@@ -188,6 +225,7 @@ class LocalGrid:
             m.cell_size[d] = gm.cellSize(d)
             m.periodic[d] = 1 if self._gg.isPeriodic(d) else 0
         m.halo_cell_width = self._h
+        m.entity = entity.code if entity is not None else 0
         return m
 
 
@@ -225,12 +263,11 @@ def createLocalMesh(local_grid):
 
 
 class ArrayLayout:
-    """Node or Cell array layout.  The interpolation and halo calls take node
-    arrays; cell arrays serve the BOV writer."""
+    """Array layout of any entity (Node, Cell, Face(d), Edge(d))."""
 
     def __init__(self, local_grid, dofs, entity):
-        if not isinstance(entity, (Node, Cell)):
-            raise NotImplementedError("cabana_amd arrays are Node or Cell arrays")
+        if not isinstance(entity, (Node, Cell, Face, Edge)):
+            raise TypeError("entity must be Node(), Cell(), Face(d) or Edge(d)")
         self._lg = local_grid
         self._dofs = int(dofs)
         self.entity = entity
@@ -259,8 +296,8 @@ class Array:
         self.label = label
         self._layout = layout
         lg = layout.localGrid()
-        extra = 1 if isinstance(layout.entity, Node) else 0
-        shape = [lg.totalNumCell(d) + extra for d in range(3)] + [layout.dofsPerEntity()]
+        shape = [lg.totalNumCell(d) + 1 - _stag(layout.entity, d) for d in range(3)] + \
+            [layout.dofsPerEntity()]
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         if memory_layout == "right":
@@ -278,6 +315,18 @@ class Array:
 
     def descriptor(self):
         return grid_descriptor(self._view)
+
+    def mesh_descriptor(self):
+        """The local grid's mesh with this array's entity."""
+        return self._layout.localGrid().mesh_descriptor(self._layout.entity)
+
+
+def _call_mesh(arrays):
+    """Mesh descriptor of a call: every array of one call has one entity."""
+    codes = {a.layout().entity.code for a in arrays}
+    if len(codes) != 1:
+        raise ValueError("the arrays of one call must share one entity")
+    return arrays[0].mesh_descriptor()
 
 
 def createArray(label, layout, dtype=torch.float64, device=None, memory_layout="right"):
@@ -448,8 +497,7 @@ class Halo:
 
     def gather(self, *arrays):
         for a in arrays or self._arrays:
-            lg = a.layout().localGrid()
-            m = lg.mesh_descriptor()
+            m = a.mesh_descriptor()
             ctx = N.context()
             g = a.descriptor()
             N.check(N.load().cabana_amd_halo_gather(ctx, ctypes.byref(m), ctypes.byref(g),
@@ -459,8 +507,7 @@ class Halo:
         if reduce_op is not ScatterReduce.Sum and not isinstance(reduce_op, ScatterReduce.Sum):
             raise NotImplementedError("cabana_amd implements ScatterReduce::Sum")
         for a in arrays or self._arrays:
-            lg = a.layout().localGrid()
-            m = lg.mesh_descriptor()
+            m = a.mesh_descriptor()
             ctx = N.context()
             g = a.descriptor()
             N.check(N.load().cabana_amd_halo_scatter_sum(ctx, ctypes.byref(m), ctypes.byref(g),
@@ -607,7 +654,7 @@ def p2g(functor, points, num_point, spline, halo, array, bins=None):
     if len(arrays) != len(functors):
         raise ValueError("one array per functor")
     lg = arrays[0].layout().localGrid()
-    m = lg.mesh_descriptor()
+    m = _call_mesh(arrays)
     pts = _points_descriptor(points, num_point)
     terms = (N.P2GTerm * len(functors))()
     for i, (f, a) in enumerate(zip(functors, arrays)):
@@ -634,7 +681,7 @@ def g2p(array, halo, points, num_point, spline, functor, bins=None):
     if len(arrays) != len(functors):
         raise ValueError("one array per functor")
     lg = arrays[0].layout().localGrid()
-    m = lg.mesh_descriptor()
+    m = _call_mesh(arrays)
     pts = _points_descriptor(points, num_point)
     terms = (N.G2PTerm * len(functors))()
     for i, (f, a) in enumerate(zip(functors, arrays)):
@@ -683,7 +730,7 @@ def g2p2g(p2g_functor, points, num_point, spline, halo, array, g2p_functor, g2p_
     ga = ga if isinstance(ga, (list, tuple)) else [ga] * len(gf)
     if len(pa) != len(pf) or len(ga) != len(gf):
         raise ValueError("one array per functor")
-    m = pa[0].layout().localGrid().mesh_descriptor()
+    m = _call_mesh(list(pa) + list(ga))
     pts = _points_descriptor(points, num_point)
     gt = _terms("g2p", gf, ga, num_point)
     pt = _terms("p2g", pf, pa, num_point)
@@ -703,7 +750,7 @@ def p2g_adjoint(functor, points, num_point, spline, halo, grid_bar, values_bar,
     gbars = grid_bar if isinstance(grid_bar, (list, tuple)) else [grid_bar]
     vbars = values_bar if isinstance(values_bar, (list, tuple)) else [values_bar]
     lg = gbars[0].layout().localGrid()
-    m = lg.mesh_descriptor()
+    m = _call_mesh(gbars)
     pts = _points_descriptor(points, num_point)
     terms = (N.P2GAdjointTerm * len(functors))()
     for i, f in enumerate(functors):
@@ -731,7 +778,7 @@ def g2p_adjoint(array, halo, points, num_point, spline, functor, values_bar, gri
     vbars = values_bar if isinstance(values_bar, (list, tuple)) else [values_bar]
     gbars = grid_bar if isinstance(grid_bar, (list, tuple)) else [grid_bar] * len(functors)
     lg = arrays[0].layout().localGrid()
-    m = lg.mesh_descriptor()
+    m = _call_mesh(list(arrays) + [g for g in gbars if g is not None])
     pts = _points_descriptor(points, num_point)
     terms = (N.G2PAdjointTerm * len(functors))()
     for i, f in enumerate(functors):
@@ -816,6 +863,8 @@ class BovWriter:
         if rest:
             gather_array = rest[0]
         layout = array.layout()
+        if not isinstance(layout.entity, (Node, Cell)):
+            raise NotImplementedError("BOV centering is nodal or zonal (Node or Cell arrays)")
         lg = layout.localGrid()
         m = lg.mesh_descriptor()
         g = grid_descriptor(array.view())

@@ -51,14 +51,22 @@ typedef enum
     CABANA_AMD_INT64 = 3
 } cabana_amd_dtype;
 
-/* Mesh entity of a grid array (Cabana_Grid_Types.hpp Node, Cell): a node
- * array spans num_cell + 2 * halo + 1 nodes per dimension, a cell array
- * num_cell + 2 * halo cells.  The interpolation entry points take node
- * arrays. */
+/* Mesh entity of a grid array (Cabana_Grid_Types.hpp Node, Cell, Face<D>,
+ * Edge<D>).  Per dimension an entity sits at the nodes (index i at
+ * ghost_low + i * dx; num_cell + 2 * halo + 1 of them) or at the cell
+ * centres (ghost_low + (i + 1/2) dx; num_cell + 2 * halo of them): Node at
+ * nodes everywhere, Cell at centres everywhere, Face<D> at nodes in D only,
+ * Edge<D> at centres in D only (Cabana_Grid_LocalMesh.hpp:176-237). */
 typedef enum
 {
     CABANA_AMD_NODE = 0,
-    CABANA_AMD_CELL = 1
+    CABANA_AMD_CELL = 1,
+    CABANA_AMD_FACE_I = 2,
+    CABANA_AMD_FACE_J = 3,
+    CABANA_AMD_FACE_K = 4,
+    CABANA_AMD_EDGE_I = 5,
+    CABANA_AMD_EDGE_J = 6,
+    CABANA_AMD_EDGE_K = 7
 } cabana_amd_entity;
 
 /* Point-to-grid operators: the functors built by createScalarValueP2G,
@@ -85,10 +93,15 @@ typedef enum
     CABANA_AMD_VECTOR_DIVERGENCE_G2P = 4
 } cabana_amd_g2p_op;
 
-/* One rank's uniform mesh, Node entity: createUniformGlobalMesh(low, high,
- * cell_size) + createGlobalGrid(periodic) + createLocalGrid(halo_cell_width)
+/* One rank's uniform mesh: createUniformGlobalMesh(low, high, cell_size) +
+ * createGlobalGrid(periodic) + createLocalGrid(halo_cell_width)
  * (Cabana_Grid_GlobalMesh.hpp:59-102, Cabana_Grid_LocalGrid.hpp:607).  The
- * number of cells per dimension is rint((high - low) / cell_size). */
+ * number of cells per dimension is rint((high - low) / cell_size).  entity
+ * (a cabana_amd_entity; 0, Node, when zero-initialised) is the entity of the
+ * grid arrays a call passes with it: their extents, owned ranges and halo
+ * bands, and the positions the splines weigh (SplineData of the array's
+ * entity, Cabana_Grid_Interpolation.hpp:874).  The cell-tiled kernels serve
+ * Node arrays; the other entities run the per-particle kernels. */
 typedef struct
 {
     double low_corner[3];
@@ -96,6 +109,7 @@ typedef struct
     double cell_size[3];
     int32_t halo_cell_width;
     int32_t periodic[3];
+    int32_t entity;
 } cabana_amd_mesh;
 
 /* Particle field (a Cabana slice or a Kokkos View indexed (p, d0[, d1])).

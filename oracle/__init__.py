@@ -33,7 +33,12 @@ class Mesh(ctypes.Structure):
     _fields_ = [("global_low", c_double * 3), ("global_high", c_double * 3),
                 ("cell_size", c_double * 3), ("num_cell", c_int32 * 3), ("halo", c_int32),
                 ("periodic", c_int32 * 3), ("ghost_low", c_double * 3),
-                ("num_node", c_int32 * 3)]
+                ("num_node", c_int32 * 3), ("entity", c_int32), ("stag", c_int32 * 3)]
+
+
+# Entity codes (0 Node, 1 Cell, 2-4 Face<I,J,K>, 5-7 Edge<I,J,K>).
+ENTITIES = {"node": 0, "cell": 1, "face_i": 2, "face_j": 3, "face_k": 4, "edge_i": 5,
+            "edge_j": 6, "edge_k": 7}
 
 
 class PField(ctypes.Structure):
@@ -68,6 +73,7 @@ def load():
         lib = ctypes.CDLL(LIB)
         lib.orc_mesh_init.argtypes = [P(Mesh), P(c_double), P(c_double), P(c_double), c_int,
                                       P(c_int)]
+        lib.orc_mesh_set_entity.argtypes = [P(Mesh), c_int]
         lib.orc_spline_map.restype = c_double
         lib.orc_spline_map.argtypes = [c_int, c_double, c_double, c_double]
         lib.orc_spline_stencil.argtypes = [c_int, c_double, P(c_int)]
@@ -121,13 +127,15 @@ class OMesh:
     """Single-rank uniform mesh: createUniformGlobalMesh + createGlobalGrid +
     createLocalGrid."""
 
-    def __init__(self, low, high, cell_size, halo=1, periodic=(True, True, True)):
+    def __init__(self, low, high, cell_size, halo=1, periodic=(True, True, True),
+                 entity="node"):
         if isinstance(cell_size, (int, float)):
             cell_size = [cell_size] * 3
         self.m = Mesh()
         per = (c_int * 3)(*[1 if p else 0 for p in periodic])
         _check(load().orc_mesh_init(ctypes.byref(self.m), _d3(low), _d3(high), _d3(cell_size),
                                     int(halo), per), "mesh init")
+        _check(load().orc_mesh_set_entity(ctypes.byref(self.m), ENTITIES[entity]), "entity")
 
     @property
     def num_node(self):
@@ -148,8 +156,8 @@ class OMesh:
         sl = []
         for d in range(3):
             h = self.m.halo
-            hi = h + self.m.num_cell[d] + (0 if self.m.periodic[d] else 1)
-            sl.append(slice(h, hi))
+            closing = 0 if (self.m.periodic[d] or self.m.stag[d]) else 1
+            sl.append(slice(h, h + self.m.num_cell[d] + closing))
         return tuple(sl)
 
 

@@ -48,16 +48,49 @@ int orc_mesh_init( orc_mesh* m, const double low[3], const double high[3],
         double own_low = low[d] + cell_size[d] * 0;
         m->ghost_low[d] = own_low - halo * cell_size[d];
         m->num_node[d] = m->num_cell[d] + 2 * halo + 1;
+        m->stag[d] = 0;
     }
     m->halo = halo;
+    m->entity = 0;
     return 0;
 }
 
-/* LocalMesh::coordinates(Node, index), grid/src/Cabana_Grid_LocalMesh.hpp:194-200 */
+/* Entity of a mesh's arrays, one dimension at a time: Cell sits at cell
+ * centres in every dimension, Face<D> at nodes in D and at cell centres in
+ * the others, Edge<D> the other way round (LocalMesh::coordinates,
+ * Cabana_Grid_LocalMesh.hpp:176-237).  A cell-centred dimension has one
+ * entity fewer (faceIndexSpace / edge and cell index spaces,
+ * Cabana_Grid_LocalGrid_impl.hpp:239-280, 534-560). */
+int orc_mesh_set_entity( orc_mesh* m, int entity )
+{
+    if ( entity < 0 || entity > 7 )
+        return 1;
+    for ( int d = 0; d < 3; ++d )
+    {
+        int s;
+        if ( entity == 0 )
+            s = 0;
+        else if ( entity == 1 )
+            s = 1;
+        else if ( entity <= 4 )
+            s = ( d == entity - 2 ) ? 0 : 1;
+        else
+            s = ( d == entity - 5 ) ? 1 : 0;
+        m->stag[d] = s;
+        m->num_node[d] = m->num_cell[d] + 2 * m->halo + 1 - s;
+    }
+    m->entity = entity;
+    return 0;
+}
+
+/* LocalMesh::coordinates(Entity, index), grid/src/Cabana_Grid_LocalMesh.hpp
+ * :176-237: ghost low + index * dx at nodes, ghost low + (index + 0.5) * dx
+ * at cell centres. */
 void orc_node_coordinates( const orc_mesh* m, const int idx[3], double x[3] )
 {
     for ( int d = 0; d < 3; ++d )
-        x[d] = m->ghost_low[d] + (double)idx[d] * m->cell_size[d];
+        x[d] = m->stag[d] ? m->ghost_low[d] + ( (double)idx[d] + 0.5 ) * m->cell_size[d]
+                          : m->ghost_low[d] + (double)idx[d] * m->cell_size[d];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -602,13 +635,15 @@ static void g2p_point( int op, double multiplier, const orc_grid* grid,
 static void owned_bounds( const orc_mesh* m, int d, int* lo, int* hi )
 {
     *lo = m->halo;
-    *hi = m->periodic[d] ? m->halo + m->num_cell[d]
-                         : m->halo + m->num_cell[d] + 1;
+    *hi = ( m->periodic[d] || m->stag[d] ) ? m->halo + m->num_cell[d]
+                                           : m->halo + m->num_cell[d] + 1;
 }
 
 /* Shared index spaces, LocalGrid_impl.hpp:470-509 with getBound :283-320:
  *   Own:   min = getBound(owned, -1, off, 0, -w), max = getBound(owned, 1, off, w+1, 0)
- *   Ghost: min = getBound(owned, -1, off, -w, 0), max = getBound(owned, 1, off, 0, w+1) */
+ *   Ghost: min = getBound(owned, -1, off, -w, 0), max = getBound(owned, 1, off, 0, w+1)
+ * for node-centred dimensions; cell-centred ones have w in place of w+1
+ * (:370-398; faces and edges mix the two per dimension). */
 static void shared_box( const orc_mesh* m, int ghost, const int off[3], int w,
                         int lo[3], int hi[3] )
 {
@@ -619,7 +654,7 @@ static void shared_box( const orc_mesh* m, int ghost, const int off[3], int w,
         if ( off[d] == -1 )
         {
             lo[d] = omin + ( ghost ? -w : 0 );
-            hi[d] = omin + ( ghost ? 0 : w + 1 );
+            hi[d] = omin + ( ghost ? 0 : w + 1 - m->stag[d] );
         }
         else if ( off[d] == 0 )
         {
@@ -629,7 +664,7 @@ static void shared_box( const orc_mesh* m, int ghost, const int off[3], int w,
         else
         {
             lo[d] = omax + ( ghost ? 0 : -w );
-            hi[d] = omax + ( ghost ? w + 1 : 0 );
+            hi[d] = omax + ( ghost ? w + 1 - m->stag[d] : 0 );
         }
     }
 }

@@ -31,7 +31,9 @@ typedef struct
     int32_t halo;         /* halo cell width h of the local grid */
     int32_t periodic[3];
     double ghost_low[3];  /* own low corner - h * cell size */
-    int32_t num_node[3];  /* ghosted node extent = num_cell + 2h + 1 */
+    int32_t num_node[3];  /* ghosted entity extent: num_cell + 2h + 1 - stag */
+    int32_t entity;       /* 0 Node, 1 Cell, 2-4 Face<I,J,K>, 5-7 Edge<I,J,K> */
+    int32_t stag[3];      /* 1 where the entity sits at cell centres */
 } orc_mesh;
 
 /* Particle field: element (p, c) lives at
@@ -172,6 +174,11 @@ int orc_lcl_build( const orc_pfield* positions, int64_t begin, int64_t end,
 /* permute(binning data, slice), core/src/Cabana_Sort.hpp:600-656 */
 int orc_permute( const int64_t* permutes, int64_t begin, int64_t end,
                  orc_pfield* field );
+
+/* Makes the mesh's arrays those of another entity (the index spaces and
+ * coordinates of grid/src/Cabana_Grid_LocalGrid_impl.hpp and
+ * Cabana_Grid_LocalMesh.hpp:176-237).  Returns 0 on success. */
+int orc_mesh_set_entity( orc_mesh* m, int entity );
 
 /* particleMigrate on one rank over row-major positions x[n][3]: -1 when
  * nothing migrates, else the kept count with keep[p] set (0 = removed). */

@@ -74,12 +74,13 @@ def min_halo(order):
 ORDERS = list(range(7))
 
 
-def random_case(oracle, order, n=9, seed=0, num_cell=(5, 4, 6), cell=0.37, halo=None):
+def random_case(oracle, order, n=9, seed=0, num_cell=(5, 4, 6), cell=0.37, halo=None,
+                entity="node"):
     rng = np.random.default_rng(seed)
-    halo = min_halo(order) if halo is None else halo
+    halo = min_halo(order) + (1 if entity != "node" else 0) if halo is None else halo
     low = np.array([0.3, -1.1, 2.0])
     high = low + cell * np.array(num_cell)
-    mesh = oracle.OMesh(low, high, cell, halo=halo)
+    mesh = oracle.OMesh(low, high, cell, halo=halo, entity=entity)
     pts = low + rng.uniform(0, 1, size=(n, 3)) * (high - low)
     return mesh, pts, rng
 
@@ -87,7 +88,10 @@ def random_case(oracle, order, n=9, seed=0, num_cell=(5, 4, 6), cell=0.37, halo=
 def dense(oracle, mesh, order, pts, derivs):
     rdx = [1.0 / ((mesh.m.ghost_low[d] + mesh.m.cell_size[d]) - mesh.m.ghost_low[d])
            for d in range(3)]
-    xnode = np.stack([(pts[:, d] - mesh.m.ghost_low[d]) * rdx[d] for d in range(3)], axis=1)
+    # Entity index units: entity i of a cell-centred dimension sits at
+    # ghost_low + (i + 1/2) dx.
+    xnode = np.stack([(pts[:, d] - mesh.m.ghost_low[d]) * rdx[d] - 0.5 * mesh.m.stag[d]
+                      for d in range(3)], axis=1)
     return dense_weights(order, xnode, mesh.num_cell, mesh.halo, rdx, derivs)
 
 
@@ -132,6 +136,21 @@ def dense_g2p(oracle, mesh, order, op, mult, pts, g_owned):
 @pytest.mark.parametrize("op", list(range(5)))
 def test_p2g_matches_dense_bruteforce(oracle, order, op):
     mesh, pts, rng = random_case(oracle, order, seed=order * 10 + op)
+    _p2g_dense_check(oracle, mesh, pts, rng, order, op)
+
+
+@pytest.mark.parametrize("entity", ["cell", "face_j", "edge_k"])
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("op", list(range(5)))
+def test_p2g_on_other_entities_matches_dense_bruteforce(oracle, order, op, entity):
+    """p2g onto Cell / Face / Edge arrays (SplineData of the array's entity,
+    Cabana_Grid_Interpolation.hpp:874; entity coordinates,
+    Cabana_Grid_LocalMesh.hpp:176-237): weights of the entity positions."""
+    mesh, pts, rng = random_case(oracle, order, seed=500 + order * 10 + op, entity=entity)
+    _p2g_dense_check(oracle, mesh, pts, rng, order, op)
+
+
+def _p2g_dense_check(oracle, mesh, pts, rng, order, op):
     nc, nd = P2G_SHAPE[op]
     vals = rng.normal(size=(len(pts), nc))
     grid = mesh.grid(nd)
@@ -145,6 +164,18 @@ def test_p2g_matches_dense_bruteforce(oracle, order, op):
 @pytest.mark.parametrize("op", list(range(5)))
 def test_g2p_matches_dense_bruteforce(oracle, order, op):
     mesh, pts, rng = random_case(oracle, order, seed=100 + order * 10 + op)
+    _g2p_dense_check(oracle, mesh, pts, rng, order, op)
+
+
+@pytest.mark.parametrize("entity", ["cell", "face_i", "edge_j"])
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("op", list(range(5)))
+def test_g2p_on_other_entities_matches_dense_bruteforce(oracle, order, op, entity):
+    mesh, pts, rng = random_case(oracle, order, seed=600 + order * 10 + op, entity=entity)
+    _g2p_dense_check(oracle, mesh, pts, rng, order, op)
+
+
+def _g2p_dense_check(oracle, mesh, pts, rng, order, op):
     nc, nd = G2P_SHAPE[op]
     grid = mesh.grid(nd)
     own = mesh.owned_slices()
@@ -327,11 +358,18 @@ def test_halo_gather_scatter_pattern(oracle, width):
     check_scatter_pattern(mesh, grid, width)
 
 
+def halo_pad(mesh, d):
+    """tstHalo3d.hpp:32-46 haloPad: 1 in a node-centred dimension, 0 in a
+    cell-centred one (Cell 0, Node 1, Face<D> 1 only in D, Edge<D> 0 only
+    in D)."""
+    return 1 - mesh.m.stag[d]
+
+
 def check_gather_pattern(mesh, grid, width, periodic):
-    """tstHalo3d.hpp:52-117 (checkGather) on one rank: after the gather a node
-    holds 1 if it is owned or lies within `width` (+1 on the high side, the
-    Node halo pad) of the owned nodes in every dimension, 0 otherwise; the
-    boundary halo of a non-periodic dimension stays 0."""
+    """tstHalo3d.hpp:52-117 (checkGather) on one rank: after the gather an
+    entity holds 1 if it is owned or lies within `width` (+ the halo pad on
+    the high side) of the owned entities in every dimension, 0 otherwise;
+    the boundary halo of a non-periodic dimension stays 0."""
     own = mesh.owned_slices()
     for i in range(mesh.num_node[0]):
         for j in range(mesh.num_node[1]):
@@ -340,7 +378,8 @@ def check_gather_pattern(mesh, grid, width, periodic):
                 boundary = any(not periodic[d] and not (own[d].start <= idx[d] < own[d].stop)
                                for d in range(3))
                 outside = any(idx[d] < own[d].start - width or
-                              idx[d] >= own[d].stop + width + 1 for d in range(3))
+                              idx[d] >= own[d].stop + width + halo_pad(mesh, d)
+                              for d in range(3))
                 want = 0.0 if (boundary or outside) else 1.0
                 assert np.all(grid[i, j, k] == want), (idx, grid[i, j, k], want)
 
@@ -357,15 +396,18 @@ def check_scatter_pattern_periodicity(mesh, grid, width, periodic):
                 for d, x in enumerate((i, j, k)):
                     if not periodic[d]:
                         continue
-                    if x < own[d].start + width + 1 or x >= own[d].stop - width:
+                    if x < own[d].start + width + halo_pad(mesh, d) or \
+                            x >= own[d].stop - width:
                         num += 1
                 assert np.all(grid[i, j, k] == 2.0 ** num), ((i, j, k), grid[i, j, k], num)
 
 
+@pytest.mark.parametrize("entity", ["node", "cell", "face_i", "face_j", "face_k", "edge_i",
+                                    "edge_j", "edge_k"])
 @pytest.mark.parametrize("periodic", [(True, True, True), (False, True, False),
                                       (True, False, True), (False, False, False)])
 @pytest.mark.parametrize("width", [1, 2, 3])
-def test_halo_gather_scatter_pattern_periodicity(oracle, width, periodic):
+def test_halo_gather_scatter_pattern_periodicity(oracle, width, periodic, entity):
     """tstHalo3d.hpp:181-230 geometry with periodic and non-periodic
     dimensions: the non-periodic branch of the oracle's halo exchange
     (no neighbour across a non-periodic boundary) against the reference's
@@ -373,7 +415,7 @@ def test_halo_gather_scatter_pattern_periodicity(oracle, width, periodic):
     cell = 0.23
     low = np.array([1.2, 3.3, -2.8])
     high = low + cell * np.array([17, 20, 21])
-    mesh = oracle.OMesh(low, high, cell, halo=3, periodic=periodic)
+    mesh = oracle.OMesh(low, high, cell, halo=3, periodic=periodic, entity=entity)
     grid = mesh.grid(4)
     grid[mesh.owned_slices()] = 1.0
     oracle.halo_gather(mesh, grid, width)
