@@ -94,6 +94,60 @@ def build_cpp_tests():
     return out
 
 
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer (device code is
+# untouched: every flag applies to the host compilation only).
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer"]
+SAN_LIB = os.path.join(HERE, "libcabana_amd_san.so")
+
+
+def _clang_runtime_dir():
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    base = os.path.join(rocm, "lib", "llvm", "lib", "clang")
+    for ver in sorted(os.listdir(base)) if os.path.isdir(base) else []:
+        d = os.path.join(base, ver, "lib", "linux")
+        if os.path.exists(os.path.join(d, "libclang_rt.asan-x86_64.so")):
+            return d
+    raise RuntimeError("clang's shared AddressSanitizer runtime was not found")
+
+
+def build_sanitized():
+    """libcabana_amd_san.so: the C-ABI host layer (ca_api.cpp, ca_no_alloc.cpp)
+    built with host AddressSanitizer and UndefinedBehaviorSanitizer, linked
+    with the kernel objects of build_library() and clang's shared sanitizer
+    runtime; and tests/cpp/test_grid_shim_san, the C++ layer's test program
+    over it.  Needs build_library() first."""
+    hipcc = _hipcc()
+    rt = _clang_runtime_dir()
+    sdir = os.path.join(BUILD, "san")
+    os.makedirs(sdir, exist_ok=True)
+    objs = []
+    for src in ("ca_api", "ca_no_alloc"):
+        obj = os.path.join(sdir, src + ".o")
+        _run([hipcc] + COMMON + SAN_HOST + ["-x", "hip", "-c", os.path.join(CSRC, src + ".cpp"),
+                                            "-o", obj])
+        objs.append(obj)
+    for name in sorted(os.listdir(BUILD)):
+        if name.endswith(".o") and name not in ("ca_api.o", "ca_no_alloc.o"):
+            objs.append(os.path.join(BUILD, name))
+    tmp = SAN_LIB + ".tmp"
+    _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+          "-Xarch_host", "-shared-libasan", "-Wl,-rpath," + rt])
+    os.replace(tmp, SAN_LIB)
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    src = os.path.join(ROOT, "tests", "cpp", "test_grid_shim.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "test_grid_shim_san")
+    _run([os.path.join(rocm, "llvm", "bin", "clang++"), "-std=c++17", "-O1", "-g",
+          "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+          "-I", os.path.join(rocm, "include"), src, "-o", out, "-L", HERE, "-lcabana_amd_san",
+          "-L", os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../cabana_amd",
+          "-Wl,-rpath," + os.path.join(rocm, "lib"), "-Wl,-rpath," + rt] + SAN_HOST +
+         ["-Xarch_host", "-shared-libasan"])
+    return SAN_LIB, out
+
+
 def build_oracle():
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-C", odir, "-s"])
@@ -105,6 +159,7 @@ def main():
     print("built", lib)
     print("built", build_oracle())
     print("built", build_cpp_tests())
+    print("built", *build_sanitized())
 
 
 if __name__ == "__main__":

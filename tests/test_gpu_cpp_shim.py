@@ -22,3 +22,25 @@ def test_cpp_grid_shim():
     print(proc.stdout)
     assert proc.returncode == 0, proc.stdout
     assert "ALL PASSED" in proc.stdout
+
+
+SAN_BIN = BIN + "_san"
+
+
+def test_cpp_grid_shim_under_asan_and_ubsan():
+    """The same program over libcabana_amd_san.so, whose C-ABI host layer is
+    built with host AddressSanitizer and UndefinedBehaviorSanitizer (device
+    code unchanged): every descriptor check, plan choice, scratch carve and
+    part-table cache path the program reaches runs instrumented.  Leak
+    reports are off because the HIP runtime keeps allocations until exit."""
+    if not os.path.exists(SAN_BIN):
+        pytest.fail("tests/cpp/test_grid_shim_san is not built (run __graft_entry__.build())")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    print("running", SAN_BIN, flush=True)
+    proc = subprocess.run([SAN_BIN], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                          timeout=300, env=env)
+    print(proc.stdout)
+    assert proc.returncode == 0, proc.stdout
+    assert "ALL PASSED" in proc.stdout
+    assert "AddressSanitizer" not in proc.stdout and "runtime error" not in proc.stdout
