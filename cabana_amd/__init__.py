@@ -13,6 +13,6 @@ from .grid import (  # noqa: F401
     createUniformGlobalMesh, createVectorDivergenceG2P, createVectorDivergenceP2G,
     createVectorGradientG2P, createVectorValueG2P, createVectorValueP2G, g2p, g2p2g, g2p_adjoint,
     grid_descriptor, p2g, p2g_adjoint, particleMigrate, particle_descriptor, profile_enable,
-    profile_report, profile_reset, set_auto_binning)
+    profile_report, profile_reset, set_auto_binning, set_deterministic)
 from .linkedcell import LinkedCellList, createLinkedCellList, permute  # noqa: F401
 from . import autograd  # noqa: F401,E402

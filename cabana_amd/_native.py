@@ -164,6 +164,7 @@ SIGNATURES = {
                                               P(ParticleField), c_int32, c_int32, c_int32,
                                               P(c_int64), P(c_int32), c_void_p]),
     "cabana_amd_set_auto_binning": (c_int32, [c_void_p, c_int32]),
+    "cabana_amd_set_deterministic": (c_int32, [c_void_p, c_int32]),
     "cabana_amd_profile_enable": (c_int32, [c_void_p, c_int32]),
     "cabana_amd_profile_reset": (c_int32, [c_void_p]),
     "cabana_amd_profile_count": (c_int32, [c_void_p, P(c_int32)]),

@@ -30,6 +30,11 @@ COMMON = [
 ]
 
 
+# roctx ranges around the entry points (rocprofiler-sdk's marker library).
+ROCTX = ["-L" + os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib"),
+         "-lrocprofiler-sdk-roctx"]
+
+
 def _hipcc():
     cand = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc")
     if os.path.exists(cand):
@@ -75,7 +80,7 @@ def build_library(verbose=False):
                 print(o)
     objs = [j[-1] for j in jobs]
     tmp = LIB + ".tmp"
-    _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ROCTX)
     os.replace(tmp, LIB)
     return LIB
 
@@ -132,7 +137,7 @@ def build_sanitized():
         if name.endswith(".o") and name not in ("ca_api.o", "ca_no_alloc.o"):
             objs.append(os.path.join(BUILD, name))
     tmp = SAN_LIB + ".tmp"
-    _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
+    _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ROCTX +
          ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
           "-Xarch_host", "-shared-libasan", "-Wl,-rpath," + rt])
     os.replace(tmp, SAN_LIB)

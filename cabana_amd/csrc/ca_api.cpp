@@ -1,6 +1,8 @@
 // C-ABI layer: descriptor validation, plan selection and kernel dispatch.
 #include "ca_internal.hpp"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -102,6 +104,8 @@ struct cabana_amd_context : public Launcher
     // (see auto_bins): counts, offsets and the particle of each binned
     // position, in SLOT_BINS.
     bool auto_binning = true;
+    // P2G through the deterministic pull kernel (cabana_amd_set_deterministic).
+    bool deterministic = false;
     cabana_amd_binning auto_bins = {};
     const int64_t* auto_perm = nullptr;
 
@@ -192,6 +196,18 @@ struct cabana_amd_context : public Launcher
 
 namespace
 {
+
+// roctx range over one entry point, named as the reference's Kokkos
+// profiling regions (Cabana_Grid_Halo_Mpi.hpp:80,170,
+// Cabana_LinkedCellList.hpp:655, Cabana_Sort.hpp:557,608) where it has them;
+// a no-op unless a tracing tool is attached.
+struct TraceRange
+{
+    explicit TraceRange( const char* name ) { roctxRangePushA( name ); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange( const TraceRange& ) = delete;
+    TraceRange& operator=( const TraceRange& ) = delete;
+};
 
 enum Slot
 {
@@ -575,6 +591,50 @@ hipError_t run_multi_term( cabana_amd_context* ctx, int order, bool fp64,
         return launch_multi_term_o5( *ctx, fp64, a, s );
     default:
         return launch_multi_term_o6( *ctx, fp64, a, s );
+    }
+}
+
+hipError_t run_pull( cabana_amd_context* ctx, int order, bool fp64, const PullArgs& a,
+                     hipStream_t s )
+{
+    switch ( order )
+    {
+    case 0:
+        return launch_p2g_pull_o0( *ctx, fp64, a, s );
+    case 1:
+        return launch_p2g_pull_o1( *ctx, fp64, a, s );
+    case 2:
+        return launch_p2g_pull_o2( *ctx, fp64, a, s );
+    case 3:
+        return launch_p2g_pull_o3( *ctx, fp64, a, s );
+    case 4:
+        return launch_p2g_pull_o4( *ctx, fp64, a, s );
+    case 5:
+        return launch_p2g_pull_o5( *ctx, fp64, a, s );
+    default:
+        return launch_p2g_pull_o6( *ctx, fp64, a, s );
+    }
+}
+
+hipError_t run_count_outside( cabana_amd_context* ctx, int order, const MeshDev& m,
+                              const PFieldDev& pts, int64_t n, hipStream_t s )
+{
+    switch ( order )
+    {
+    case 0:
+        return launch_count_outside_o0( *ctx, m, pts, n, ctx->skipped, s );
+    case 1:
+        return launch_count_outside_o1( *ctx, m, pts, n, ctx->skipped, s );
+    case 2:
+        return launch_count_outside_o2( *ctx, m, pts, n, ctx->skipped, s );
+    case 3:
+        return launch_count_outside_o3( *ctx, m, pts, n, ctx->skipped, s );
+    case 4:
+        return launch_count_outside_o4( *ctx, m, pts, n, ctx->skipped, s );
+    case 5:
+        return launch_count_outside_o5( *ctx, m, pts, n, ctx->skipped, s );
+    default:
+        return launch_count_outside_o6( *ctx, m, pts, n, ctx->skipped, s );
     }
 }
 
@@ -1083,6 +1143,9 @@ cabana_amd_status validate_bins( cabana_amd_context* ctx,
 // the tiled kernels read particles through (the particles stay where they
 // are).  `bins` is left as it is when the call passed one, when the
 // context has the feature off, or when it cannot apply.
+cabana_amd_status build_library_bins( cabana_amd_context* ctx, const MeshDev& m,
+                                      const PFieldDev& pts, int64_t n, hipStream_t s );
+
 cabana_amd_status auto_bins( cabana_amd_context* ctx, const MeshDev& m, int order,
                              const PFieldDev& pts, int64_t n,
                              const cabana_amd_binning*& bins, hipStream_t s )
@@ -1093,6 +1156,19 @@ cabana_amd_status auto_bins( cabana_amd_context* ctx, const MeshDev& m, int orde
     const int64_t ncell = (int64_t)m.num_cell[0] * m.num_cell[1] * m.num_cell[2];
     if ( ncell >= INT32_MAX / 2 )
         return CABANA_AMD_OK;
+    CA_CHECK( build_library_bins( ctx, m, pts, n, s ) );
+    bins = &ctx->auto_bins;
+    return CABANA_AMD_OK;
+}
+
+// The library's own cell binning of particles [0, n) over the owned cells
+// of a Node mesh (keys clamped into them), into ctx->auto_bins / auto_perm.
+cabana_amd_status build_library_bins( cabana_amd_context* ctx, const MeshDev& m,
+                                      const PFieldDev& pts, int64_t n, hipStream_t s )
+{
+    const int64_t ncell = (int64_t)m.num_cell[0] * m.num_cell[1] * m.num_cell[2];
+    if ( ncell >= INT32_MAX / 2 )
+        return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED, "more than 2^30 cells" );
     int key_bits = 0;
     while ( ( ( ncell - 1 ) >> key_bits ) != 0 )
         ++key_bits;
@@ -1133,7 +1209,6 @@ cabana_amd_status auto_bins( cabana_amd_context* ctx, const MeshDev& m, int orde
     for ( int d = 0; d < 3; ++d )
         ctx->auto_bins.num_bin[d] = m.num_cell[d];
     ctx->auto_perm = perm;
-    bins = &ctx->auto_bins;
     return CABANA_AMD_OK;
 }
 
@@ -1259,7 +1334,7 @@ cabana_amd_status p2g_impl( cabana_amd_context* ctx, const cabana_amd_mesh* mesh
                      "need points and 1..16 terms" );
     const int w = resolve_width( m, halo_width );
     CA_CHECK( check_width( ctx, m, w ) );
-    const int64_t n = points->num_particle;
+    int64_t n = points->num_particle;
     CA_CHECK( validate_bins( ctx, bins, n ) );
     bool fp64 = true;
     CA_CHECK( precision_of( ctx, terms[0].values.dtype, fp64 ) );
@@ -1293,6 +1368,33 @@ cabana_amd_status p2g_impl( cabana_amd_context* ctx, const cabana_amd_mesh* mesh
                 return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                              "terms share grid memory with different layouts" );
     hipStream_t s = static_cast<hipStream_t>( stream );
+    if ( ctx->deterministic && !fixed )
+    {
+        // Deterministic: the library's own binning of every particle, then
+        // one pull pass per term (each node sums its particles in a fixed
+        // order and is written once); no atomics anywhere.
+        if ( !node_arrays( m ) )
+            return fail( ctx, CABANA_AMD_ERROR_UNSUPPORTED,
+                         "deterministic P2G takes Node arrays" );
+        if ( n > 0 )
+        {
+            CA_CHECK( build_library_bins( ctx, m, a.points, n, s ) );
+            CA_HIP( ctx, run_count_outside( ctx, order, m, a.points, n, s ), "domain check" );
+            for ( int i = 0; i < num_terms; ++i )
+            {
+                PullArgs pa;
+                std::memset( &pa, 0, sizeof( pa ) );
+                pa.mesh = m;
+                pa.points = a.points;
+                pa.term = multi_term( T[i] );
+                pa.counts = ctx->auto_bins.counts;
+                pa.offsets = ctx->auto_bins.offsets;
+                pa.perm = ctx->auto_perm;
+                CA_HIP( ctx, run_pull( ctx, order, fp64, pa, s ), "p2g pull" );
+            }
+        }
+        n = 0; // nothing left for the other paths; the fold below still runs
+    }
     if ( !fixed )
         CA_CHECK( auto_bins( ctx, m, order, a.points, n, bins, s ) );
     if ( n > 0 )
@@ -1376,6 +1478,7 @@ extern "C" cabana_amd_status cabana_amd_p2g(
     const cabana_amd_p2g_term* terms, int32_t halo_width,
     const cabana_amd_binning* bins, cabana_amd_stream stream )
 {
+    TraceRange trace_range( "Cabana::Grid::p2g" );
     return p2g_impl( ctx, mesh, order, points, num_terms, terms, halo_width, bins, stream,
                      nullptr, 0 );
 }
@@ -1481,6 +1584,7 @@ extern "C" cabana_amd_status cabana_amd_g2p(
     const cabana_amd_g2p_term* terms, int32_t halo_width,
     const cabana_amd_binning* bins, cabana_amd_stream stream )
 {
+    TraceRange trace_range( "Cabana::Grid::g2p" );
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
     MeshDev m;
@@ -1562,12 +1666,21 @@ extern "C" cabana_amd_status cabana_amd_g2p2g(
     const cabana_amd_p2g_term* p2g_terms, int32_t halo_width,
     const cabana_amd_binning* bins, cabana_amd_stream stream )
 {
+    TraceRange trace_range( "Cabana::Grid::g2p2g" );
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
     if ( !points || num_g2p_terms < 1 || num_g2p_terms > MAX_CALL_TERMS || !g2p_terms ||
          num_p2g_terms < 1 || num_p2g_terms > MAX_CALL_TERMS || !p2g_terms )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
                      "need points, 1..16 G2P terms and 1..16 P2G terms" );
+    if ( ctx->deterministic )
+    {
+        // Every G2P read before any P2G write; the P2G is the deterministic one.
+        CA_CHECK( cabana_amd_g2p( ctx, mesh, order, points, num_g2p_terms, g2p_terms, halo_width,
+                                  bins, stream ) );
+        return cabana_amd_p2g( ctx, mesh, order, points, num_p2g_terms, p2g_terms, halo_width,
+                               bins, stream );
+    }
     if ( !bins && ctx->auto_binning && order >= 1 && order <= 3 )
     {
         // The library's own binning, built once for both halves.
@@ -1714,6 +1827,7 @@ extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
     const cabana_amd_particle_field* points_bar,
     const cabana_amd_binning* bins, cabana_amd_stream stream )
 {
+    TraceRange trace_range( "cabana_amd::p2g_adjoint" );
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
     MeshDev m;
@@ -1803,6 +1917,7 @@ extern "C" cabana_amd_status cabana_amd_g2p_adjoint(
     const cabana_amd_particle_field* points_bar,
     const cabana_amd_binning* bins, cabana_amd_stream stream )
 {
+    TraceRange trace_range( "cabana_amd::g2p_adjoint" );
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
     MeshDev m;
@@ -1991,6 +2106,7 @@ extern "C" cabana_amd_status cabana_amd_halo_gather(
     const cabana_amd_grid_field* grid, int32_t halo_width,
     cabana_amd_stream stream )
 {
+    TraceRange trace_range( "Cabana::Grid::gather" );
     MeshDev m;
     GFieldDev g;
     int w;
@@ -2008,6 +2124,7 @@ extern "C" cabana_amd_status cabana_amd_halo_scatter_sum(
     const cabana_amd_grid_field* grid, int32_t halo_width,
     cabana_amd_stream stream )
 {
+    TraceRange trace_range( "Cabana::Grid::scatter" );
     MeshDev m;
     GFieldDev g;
     int w;
@@ -2070,6 +2187,7 @@ extern "C" cabana_amd_status cabana_amd_linked_cell_build(
     const double grid_min[3], const double grid_max[3], int32_t* counts,
     int64_t* offsets, int64_t* permutes, cabana_amd_stream stream )
 {
+    TraceRange trace_range( "Cabana::LinkedCellList::build" );
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
     if ( !positions || !grid_delta || !grid_min || !grid_max || !counts ||
@@ -2151,6 +2269,7 @@ extern "C" cabana_amd_status cabana_amd_permute(
     int64_t end, const cabana_amd_particle_field* field,
     cabana_amd_stream stream )
 {
+    TraceRange trace_range( "Cabana::permute" );
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
     if ( !field || ( !permutes && end > begin ) )
@@ -2182,6 +2301,7 @@ extern "C" cabana_amd_status cabana_amd_permute_members(
     const cabana_amd_particle_field* fields, int32_t num_fields,
     cabana_amd_stream stream )
 {
+    TraceRange trace_range( "Cabana::permute" );
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
     if ( num_fields < 0 || ( num_fields > 0 && !fields ) || ( !permutes && end > begin ) )
@@ -2221,6 +2341,7 @@ extern "C" cabana_amd_status cabana_amd_bov_write_time_step(
     int32_t entity, const char* label, const char* prefix, int32_t time_step_index,
     double time, int32_t gather_array, cabana_amd_stream stream )
 {
+    TraceRange trace_range( "Cabana::Grid::BovWriter::writeTimeStep" );
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
     if ( !grid || !label || !prefix )
@@ -2338,6 +2459,7 @@ extern "C" cabana_amd_status cabana_amd_particle_migrate(
     int32_t num_members, int32_t min_halo_width, int32_t force_migrate,
     int64_t* num_particle, int32_t* migrated, cabana_amd_stream stream )
 {
+    TraceRange trace_range( "Cabana::Grid::particleMigrate" );
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
     if ( !positions || !num_particle || !migrated || num_members < 0 ||
@@ -2444,6 +2566,15 @@ extern "C" cabana_amd_status cabana_amd_particle_migrate(
 // ---------------------------------------------------------------------------
 // Profiling
 // ---------------------------------------------------------------------------
+
+extern "C" cabana_amd_status cabana_amd_set_deterministic( cabana_amd_context* ctx,
+                                                           int32_t enable )
+{
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    ctx->deterministic = enable != 0;
+    return CABANA_AMD_OK;
+}
 
 extern "C" cabana_amd_status cabana_amd_set_auto_binning( cabana_amd_context* ctx,
                                                           int32_t enable )

@@ -312,6 +312,22 @@ static_assert( sizeof( MultiTermArgs ) <= 4000, "MultiTermArgs exceeds the kerne
 inline MultiTermDev multi_term( const GatherTermDev& t ) { return { t.pout, t.grid, t.mult, t.form }; }
 inline MultiTermDev multi_term( const ScatterTermDev& t ) { return { t.pin, t.grid, t.mult, t.form }; }
 
+// Deterministic P2G of one term: every node of the term's grid sums the
+// contributions of the particles of the cells around it, cell by cell in
+// cardinal order and in binned order inside a cell, and adds the sum to
+// itself (one writer per node, no atomics).  The binning covers every
+// particle (cell keys clamped into the owned cells) and lists them through
+// perm.
+struct PullArgs
+{
+    MeshDev mesh;
+    PFieldDev points;
+    MultiTermDev term;
+    const int32_t* counts;
+    const int64_t* offsets;
+    const int64_t* perm;
+};
+
 // Brick tiling of cell-sorted particles.  A brick is tile[0] x tile[1] x
 // tile[2] mesh cells; its particles are the tile[0] * tile[1] cell runs along
 // the last dimension (contiguous in the cell-sorted order).  Bricks are cut
@@ -391,7 +407,13 @@ enum Plan : int32_t
                                                int plan, const ScatterArgs& a, \
                                                hipStream_t s );                \
     hipError_t launch_multi_term_##SUFFIX( Launcher& L, bool fp64,             \
-                                           const MultiTermArgs& a, hipStream_t s );
+                                           const MultiTermArgs& a, hipStream_t s ); \
+    hipError_t launch_p2g_pull_##SUFFIX( Launcher& L, bool fp64, const PullArgs& a, \
+                                         hipStream_t s );                      \
+    hipError_t launch_count_outside_##SUFFIX( Launcher& L, const MeshDev& m,   \
+                                              const PFieldDev& pts, int64_t n, \
+                                              unsigned long long* skipped,     \
+                                              hipStream_t s );
 #define CABANA_AMD_DECLARE_TILED( SUFFIX )                                     \
     hipError_t launch_gather_tiled_##SUFFIX( Launcher& L, bool fp64, int plan, \
                                              const GatherArgs& a,              \

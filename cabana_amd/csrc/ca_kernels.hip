@@ -516,6 +516,162 @@ hipError_t scatter_atomic_typed( Launcher& L, int plan, const ScatterArgs& a,
     return e;
 }
 
+// ---------------------------------------------------------------------------
+// Deterministic P2G ("pull"): one thread per node of the grid
+// ---------------------------------------------------------------------------
+
+// Weight and gradient of stencil node j (0 <= j < K) of a 1-D spline, the
+// node chosen at run time from compile-time unrolled candidates.
+template <class Real>
+__device__ __forceinline__ void pick_node( const Spline1D<ORDER, Real>& s, int j, Real& w,
+                                           Real& g )
+{
+    w = Real( 0 );
+    g = Real( 0 );
+    static_for<K>( [&]( auto J ) {
+        constexpr int jj = J;
+        w = ( j == jj ) ? s.w[jj] : w;
+        g = ( j == jj ) ? s.g[jj] : g;
+    } );
+}
+
+template <class Real, int FORM>
+__global__ __launch_bounds__( BLOCK ) void p2g_pull_kernel( PullArgs a )
+{
+    constexpr int NP = FormTraits<FORM>::NP;
+    constexpr int NG = FormTraits<FORM>::NG;
+    const MeshDev& m = a.mesh;
+    const GFieldDev& g = a.term.grid;
+    const int64_t total = (int64_t)g.ext[0] * g.ext[1] * g.ext[2];
+    const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( e >= total )
+        return;
+    const int nd[3] = { (int)( e / ( (int64_t)g.ext[1] * g.ext[2] ) ),
+                        (int)( ( e / g.ext[2] ) % g.ext[1] ), (int)( e % g.ext[2] ) };
+    // A particle of owned cell c has its stencil base at c + h - ORDER / 2
+    // (or one more, for the dual grid of even orders) and reaches K nodes, so
+    // node n sees cells [n - h + ORDER / 2 - K, n - h + ORDER / 2].  Particles
+    // outside the owned cells are binned into the boundary cell nearest to
+    // them, so the range is clamped into the owned cells without becoming
+    // empty; every candidate is checked against its actual stencil.
+    int lo[3], hi[3];
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        const int N = m.num_cell[d];
+        const int top = nd[d] - m.halo + ORDER / 2;
+        lo[d] = min( max( top - K, 0 ), N - 1 );
+        hi[d] = min( max( top, 0 ), N - 1 );
+    }
+    const Real mult = (Real)a.term.mult;
+    double acc[NG];
+#pragma unroll
+    for ( int c = 0; c < NG; ++c )
+        acc[c] = 0.0;
+    for ( int ci = lo[0]; ci <= hi[0]; ++ci )
+        for ( int cj = lo[1]; cj <= hi[1]; ++cj )
+            for ( int ck = lo[2]; ck <= hi[2]; ++ck )
+            {
+                const int64_t cell = ( (int64_t)ci * m.num_cell[1] + cj ) * m.num_cell[2] + ck;
+                const int64_t q0 = a.offsets[cell];
+                const int64_t q1 = q0 + a.counts[cell];
+                for ( int64_t q = q0; q < q1; ++q )
+                {
+                    const int64_t p = a.perm[q];
+                    double px[3];
+                    load_position( a.points, p, px );
+                    Real w[3], gr[3];
+                    bool hit = true;
+#pragma unroll
+                    for ( int d = 0; d < 3; ++d )
+                    {
+                        const double x0 = logical_coord<ORDER>( px[d], m.low[d], m.rdx[d] );
+                        int base;
+                        Real f;
+                        Spline1D<ORDER, Real> s1;
+                        if ( !spline_frac<ORDER, Real>( x0, m.num_node[d], base, f ) )
+                        {
+                            hit = false;
+                            w[d] = gr[d] = Real( 0 );
+                            continue;
+                        }
+                        const int j = nd[d] - base;
+                        hit = hit && j >= 0 && j < K;
+                        spline_weights<ORDER, Real, false>( f, (Real)m.rdx[d], s1 );
+                        pick_node( s1, j, w[d], gr[d] );
+                    }
+                    if ( !hit )
+                        continue;
+                    NodeW<Real, false> nw;
+                    const Real wyz = w[1] * w[2];
+                    nw.W = w[0] * wyz;
+                    nw.dW[0] = gr[0] * wyz;
+                    nw.dW[1] = w[0] * gr[1] * w[2];
+                    nw.dW[2] = w[0] * w[1] * gr[2];
+                    const uint32_t b = a.term.f.base( p );
+                    Real Pm[NP];
+#pragma unroll
+                    for ( int c = 0; c < NP; ++c )
+                        Pm[c] = *a.term.f.template ptr<const Real>( b, c ) * mult;
+                    Real G[NG];
+                    form_scatter<FORM>( Pm, nw, G );
+#pragma unroll
+                    for ( int c = 0; c < NG; ++c )
+                        acc[c] += (double)G[c];
+                }
+            }
+    Real* dst = gptr<Real>( g, nd[0] * g.s[0] + nd[1] * g.s[1] + nd[2] * g.s[2] );
+#pragma unroll
+    for ( int c = 0; c < NG; ++c )
+        if ( acc[c] != 0.0 )
+            dst[c * g.s[3]] += (Real)acc[c];
+}
+
+// Particles whose stencil leaves the ghosted grid (counted once each, as
+// the other paths count them).
+__global__ __launch_bounds__( BLOCK ) void count_outside_kernel( MeshDev m, PFieldDev pts,
+                                                                 int64_t n,
+                                                                 unsigned long long* skipped )
+{
+    const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if ( p >= n )
+        return;
+    double px[3];
+    load_position( pts, p, px );
+    int b[3];
+    const bool out = !stencil_base<ORDER>( m, px, b );
+    const unsigned long long mask = __ballot( out );
+    if ( mask && (int)( threadIdx.x & 63 ) == __ffsll( (long long)mask ) - 1 )
+        atomicAdd( skipped, (unsigned long long)__popcll( mask ) );
+}
+
+template <class Real>
+hipError_t pull_typed( Launcher& L, const PullArgs& a, hipStream_t s )
+{
+    const int64_t total = (int64_t)a.term.grid.ext[0] * a.term.grid.ext[1] * a.term.grid.ext[2];
+    if ( total <= 0 )
+        return hipSuccess;
+    L.before( "p2g_pull", s );
+    switch ( a.term.form )
+    {
+#define CA_CASE( F )                                                           \
+    case F:                                                                    \
+        hipLaunchKernelGGL( ( p2g_pull_kernel<Real, F> ), blocks_for( total ), dim3( BLOCK ), 0, \
+                            s, a );                                            \
+        break;
+        CA_CASE( FORM_SV )
+        CA_CASE( FORM_VV )
+        CA_CASE( FORM_SG )
+        CA_CASE( FORM_VD )
+        CA_CASE( FORM_TD )
+#undef CA_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    L.after( "p2g_pull", s );
+    return hipGetLastError();
+}
+
 template <class Real>
 hipError_t multi_term_typed( Launcher& L, const MultiTermArgs& a, hipStream_t s )
 {
@@ -530,6 +686,26 @@ hipError_t multi_term_typed( Launcher& L, const MultiTermArgs& a, hipStream_t s 
 }
 
 } // namespace
+
+hipError_t CA_CAT( launch_p2g_pull_, CA_SUFFIX )( Launcher& L, bool fp64, const PullArgs& a,
+                                                 hipStream_t s )
+{
+    return fp64 ? pull_typed<double>( L, a, s ) : pull_typed<float>( L, a, s );
+}
+
+hipError_t CA_CAT( launch_count_outside_, CA_SUFFIX )( Launcher& L, const MeshDev& m,
+                                                      const PFieldDev& pts, int64_t n,
+                                                      unsigned long long* skipped,
+                                                      hipStream_t s )
+{
+    if ( n <= 0 )
+        return hipSuccess;
+    L.before( "count_outside", s );
+    hipLaunchKernelGGL( count_outside_kernel, blocks_for( n ), dim3( BLOCK ), 0, s, m, pts, n,
+                        skipped );
+    L.after( "count_outside", s );
+    return hipGetLastError();
+}
 
 hipError_t CA_CAT( launch_multi_term_, CA_SUFFIX )( Launcher& L, bool fp64,
                                                    const MultiTermArgs& a, hipStream_t s )

@@ -922,3 +922,11 @@ def set_auto_binning(enable=True):
     kernels with global atomics."""
     ctx = N.context()
     N.check(N.load().cabana_amd_set_auto_binning(ctx, 1 if enable else 0), ctx)
+
+
+def set_deterministic(enable=True):
+    """Deterministic P2G: every P2G call bins the particles and sums each
+    node's contributions in a fixed order, so repeated calls give bitwise
+    equal grids (Node arrays; slower; off by default)."""
+    ctx = N.context()
+    N.check(N.load().cabana_amd_set_deterministic(ctx, 1 if enable else 0), ctx)

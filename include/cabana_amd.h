@@ -481,6 +481,18 @@ cabana_amd_status cabana_amd_particle_migrate(
  * global atomics); it is on by default.  No reference counterpart. */
 cabana_amd_status cabana_amd_set_auto_binning( cabana_amd_context* ctx, int32_t enable );
 
+/* Deterministic P2G (enable = 1; off by default).  cabana_amd_p2g and the
+ * P2G half of cabana_amd_g2p2g then bin the particles themselves and run a
+ * pull pass per term: every node of the grid sums the contributions of the
+ * particles of the cells around it, cell by cell in cardinal order and in
+ * binned order inside a cell, in FP64, and is written once; the fold is
+ * deterministic too.  The same inputs then give bitwise the same grids
+ * from call to call (the reference's atomic ScatterView does not).  Node
+ * arrays only (CABANA_AMD_ERROR_UNSUPPORTED otherwise); slower than the
+ * default path; the adjoint scatters and cabana_amd_p2g_no_alloc are not
+ * affected.  No reference counterpart. */
+cabana_amd_status cabana_amd_set_deterministic( cabana_amd_context* ctx, int32_t enable );
+
 /* ----------------------------------------------------------------------- */
 /* Kernel timing (HIP events around every launch of this context)           */
 /* ----------------------------------------------------------------------- */
