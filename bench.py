@@ -75,6 +75,8 @@ def algorithmic_bytes(kernel, n, s, nodes):
         return n * (24 + 12 * s) + nodes * 3 * s * 2
     if kernel.startswith("p2g_adjoint_mpm"):    # gbar, m, q, sigma -> bars +=, xbar +=
         return n * (24 + 13 * s + 2 * 13 * s + 48) + nodes * 7 * s
+    if kernel.startswith("adjoint_mpm"):        # the above with vbar, Lbar and u read too
+        return n * (24 + 13 * s + 2 * 13 * s + 12 * s + 48) + nodes * 10 * s
     return None
 
 
@@ -151,12 +153,15 @@ class Workload:
                [self.gmass, self.gmom, self.gstress], bins=self.bins)
         ca.g2p([self.u, self.u], self.halo, self.pos, self.n, self.sp, self.g2p_f,
                bins=self.bins)
+        # Adjoint: the G2P adjoint's grid part (a scatter), then one gather
+        # for the P2G adjoint and both position cotangents.
         ca.g2p_adjoint([self.u, self.u], self.halo, self.pos, self.n, self.sp, self.g2p_adj_f,
-                       [self.ubar_v, self.ubar_L], [self.gbar_u, self.gbar_u], self.xbar,
+                       [self.ubar_v, self.ubar_L], [self.gbar_u, self.gbar_u], None,
                        bins=self.bins)
-        ca.p2g_adjoint(self.p2g_f, self.pos, self.n, self.sp, self.halo,
-                       [self.gbar_m, self.gbar_q, self.gbar_s],
-                       [self.mbar, self.qbar, self.sbar], self.xbar, bins=self.bins)
+        ca.adjoint_gather(self.p2g_f, self.u, self.halo, self.pos, self.n, self.sp,
+                          [self.gbar_m, self.gbar_q, self.gbar_s],
+                          [self.mbar, self.qbar, self.sbar], self.g2p_adj_f,
+                          [self.ubar_v, self.ubar_L], self.xbar, bins=self.bins)
 
     def outputs_of_one_step(self):
         """Every output of one step from zeroed accumulators, as FP64 device

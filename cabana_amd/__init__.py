@@ -4,6 +4,7 @@ p2g / g2p operator semantics.  The compute runs in hand-written HIP kernels
 interface over the C ABI."""
 from .grid import (  # noqa: F401
     AoSoASlice, Array, ArrayLayout, ArrayOp, BovWriter, Cell, Dim, Edge, Experimental, Face, Ghost,
+    adjoint_gather,
     GlobalGrid,
     GlobalMesh, Halo,
     IndexSpace, Local, LocalGrid, LocalMesh, Node, NodeHaloPattern, Own, ScatterReduce,

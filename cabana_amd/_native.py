@@ -145,6 +145,10 @@ SIGNATURES = {
     "cabana_amd_g2p_adjoint": (c_int32, [c_void_p, P(Mesh), c_int32, P(ParticleField),
                                          c_int32, P(G2PAdjointTerm), c_int32,
                                          P(ParticleField), P(Binning), c_void_p]),
+    "cabana_amd_adjoint_gather": (c_int32, [c_void_p, P(Mesh), c_int32, P(ParticleField),
+                                            c_int32, P(P2GAdjointTerm), c_int32,
+                                            P(G2PAdjointTerm), c_int32, P(ParticleField),
+                                            P(Binning), c_void_p]),
     "cabana_amd_halo_gather": (c_int32, [c_void_p, P(Mesh), P(GridField), c_int32, c_void_p]),
     "cabana_amd_halo_scatter_sum": (c_int32, [c_void_p, P(Mesh), P(GridField), c_int32,
                                               c_void_p]),

@@ -995,6 +995,13 @@ int slot_dofs( int plan, const int* forms, int nterms, int* ng )
         ng[0] = 3;
         return 1;
     }
+    if ( plan == PLAN_MPM_ADJOINT )
+    {
+        // Three cotangent grids, then the shared forward grid.
+        ng[0] = 1;
+        ng[1] = ng[2] = ng[3] = 3;
+        return 4;
+    }
     for ( int i = 0; i < nterms; ++i )
         ng[i] = form_gdof( forms[i] );
     return nterms;
@@ -1066,7 +1073,7 @@ cabana_amd_status gather_pass( cabana_amd_context* ctx, int order, bool fp64,
                                int plan, GatherArgs a, int64_t n,
                                const cabana_amd_binning* bins, hipStream_t s )
 {
-    int forms[MAX_TERMS], ng[MAX_TERMS];
+    int forms[MAX_GATHER_TERMS], ng[MAX_GATHER_TERMS];
     for ( int i = 0; i < a.nterms; ++i )
         forms[i] = a.t[i].form;
     const int nslot = slot_dofs( plan, forms, a.nterms, ng );
@@ -1820,14 +1827,24 @@ extern "C" cabana_amd_status cabana_amd_g2p2g(
 // Adjoints
 // ---------------------------------------------------------------------------
 
-extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
-    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
-    const cabana_amd_particle_field* points, int32_t num_terms,
-    const cabana_amd_p2g_adjoint_term* terms, int32_t halo_width,
-    const cabana_amd_particle_field* points_bar,
-    const cabana_amd_binning* bins, cabana_amd_stream stream )
+namespace
 {
-    TraceRange trace_range( "cabana_amd::p2g_adjoint" );
+// cabana_amd_p2g_adjoint, and with num_gx > 0 the position part of the G2P
+// adjoint terms gx as well.  Those are taken into the pass (and *fused set)
+// only for the MPM set: P2G (ScalarValue, VectorValue, TensorDivergence)
+// with G2P (VectorValue, VectorGradient) on one grid that is none of the
+// cotangent grids; otherwise *fused stays false and the caller runs them.
+cabana_amd_status p2g_adjoint_impl( cabana_amd_context* ctx, const cabana_amd_mesh* mesh,
+                                    int32_t order, const cabana_amd_particle_field* points,
+                                    int32_t num_terms, const cabana_amd_p2g_adjoint_term* terms,
+                                    int32_t halo_width,
+                                    const cabana_amd_particle_field* points_bar,
+                                    const cabana_amd_binning* bins, cabana_amd_stream stream,
+                                    int32_t num_gx, const cabana_amd_g2p_adjoint_term* gx,
+                                    bool* fused )
+{
+    if ( fused )
+        *fused = false;
     if ( !ctx )
         return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
     MeshDev m;
@@ -1892,6 +1909,40 @@ extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
                      !pfields_overlap( T[0].pout, T[1].pout, n, fp64 ? 8 : 4 ) &&
                      !pfields_overlap( T[0].pout, T[2].pout, n, fp64 ? 8 : 4 ) &&
                      !pfields_overlap( T[1].pout, T[2].pout, n, fp64 ? 8 : 4 );
+    if ( mpm && num_gx == 2 && gx && fused && gx[0].op == CABANA_AMD_VECTOR_VALUE_G2P &&
+         gx[1].op == CABANA_AMD_VECTOR_GRADIENT_G2P )
+    {
+        // The position part of the G2P adjoint of (VectorValue,
+        // VectorGradient) as two position-only terms of the same gather:
+        // the output cotangents contracted with the forward grid's jets.
+        GatherTermDev X[2];
+        std::memset( X, 0, sizeof( X ) );
+        for ( int i = 0; i < 2; ++i )
+        {
+            const int form = g2p_form( gx[i].op );
+            X[i].form = form;
+            X[i].mult = gx[i].multiplier;
+            CA_CHECK( make_pfield( ctx, gx[i].values_bar, form_pcomp( form ), rdt, n, false,
+                                   "values_bar", X[i].px ) );
+            CA_CHECK( make_gfield( ctx, m, gx[i].grid, form_gdof( form ), rdt, false, "grid",
+                                   X[i].grid ) );
+        }
+        bool own_grid = same_grid( X[0].grid, X[1].grid );
+        for ( int i = 0; i < num_terms; ++i )
+            own_grid = own_grid && T[i].grid.data != X[0].grid.data;
+        if ( own_grid )
+        {
+            // Forward ghost fill of the grid read for the position cotangent.
+            CA_HIP( ctx, launch_halo_gather( *ctx, fp64, m, X[0].grid, w, s ), "halo gather" );
+            a.nterms = 5;
+            for ( int i = 0; i < 3; ++i )
+                a.t[i] = T[i];
+            a.t[3] = X[0];
+            a.t[4] = X[1];
+            *fused = true;
+            return gather_pass( ctx, order, fp64, PLAN_MPM_ADJOINT, a, n, bins, s );
+        }
+    }
     if ( num_terms == 1 || mpm )
     {
         a.nterms = num_terms;
@@ -1908,6 +1959,53 @@ extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
         CA_CHECK( gather_pass( ctx, order, fp64, PLAN_SINGLE, b, n, bins, s ) );
     }
     return CABANA_AMD_OK;
+}
+
+} // namespace
+
+extern "C" cabana_amd_status cabana_amd_p2g_adjoint(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
+    const cabana_amd_particle_field* points, int32_t num_terms,
+    const cabana_amd_p2g_adjoint_term* terms, int32_t halo_width,
+    const cabana_amd_particle_field* points_bar,
+    const cabana_amd_binning* bins, cabana_amd_stream stream )
+{
+    TraceRange trace_range( "cabana_amd::p2g_adjoint" );
+    return p2g_adjoint_impl( ctx, mesh, order, points, num_terms, terms, halo_width, points_bar,
+                             bins, stream, 0, nullptr, nullptr );
+}
+
+extern "C" cabana_amd_status cabana_amd_adjoint_gather(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t order,
+    const cabana_amd_particle_field* points, int32_t num_p2g_terms,
+    const cabana_amd_p2g_adjoint_term* p2g_terms, int32_t num_g2p_terms,
+    const cabana_amd_g2p_adjoint_term* g2p_terms, int32_t halo_width,
+    const cabana_amd_particle_field* points_bar, const cabana_amd_binning* bins,
+    cabana_amd_stream stream )
+{
+    TraceRange trace_range( "cabana_amd::adjoint_gather" );
+    if ( !ctx )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT, "ctx is NULL" );
+    if ( !points || !points_bar || !points_bar->data || num_p2g_terms < 1 ||
+         num_p2g_terms > MAX_CALL_TERMS || !p2g_terms || num_g2p_terms < 1 ||
+         num_g2p_terms > MAX_CALL_TERMS || !g2p_terms )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "need points, points_bar, 1..16 P2G terms and 1..16 G2P terms" );
+    bool fused = false;
+    CA_CHECK( p2g_adjoint_impl( ctx, mesh, order, points, num_p2g_terms, p2g_terms, halo_width,
+                                points_bar, bins, stream, num_g2p_terms, g2p_terms, &fused ) );
+    if ( fused )
+        return CABANA_AMD_OK;
+    // Otherwise the G2P adjoint's position part as its own passes: the
+    // terms without their grid cotangents.
+    cabana_amd_g2p_adjoint_term gt[MAX_CALL_TERMS];
+    for ( int i = 0; i < num_g2p_terms; ++i )
+    {
+        gt[i] = g2p_terms[i];
+        std::memset( &gt[i].grid_bar, 0, sizeof( gt[i].grid_bar ) );
+    }
+    return cabana_amd_g2p_adjoint( ctx, mesh, order, points, num_g2p_terms, gt, halo_width,
+                                   points_bar, bins, stream );
 }
 
 extern "C" cabana_amd_status cabana_amd_g2p_adjoint(

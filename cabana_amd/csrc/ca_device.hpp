@@ -668,23 +668,56 @@ enum GatherMode : int
 constexpr bool mode_xbar( int mode ) { return mode != GATHER_OUT; }
 constexpr bool mode_out( int mode ) { return mode != GATHER_XBAR; }
 
-// Jet levels every grid slot of a gather pass needs (SHARED: one slot read
-// by every term): values for value-type outputs, gradients for gradient-type
-// outputs and value-type position cotangents, Hessians for gradient-type
-// position cotangents.
+// Flags on the entries of a gather pass's form list.  FORM_XONLY: the term
+// adds only its position cotangent (the position part of a G2P adjoint run
+// beside a P2G adjoint); its particle field is the contraction vector and
+// it has no output.  FORM_SHARE_PREV: the term reads the grid slot of the
+// term before it.
+constexpr int FORM_XONLY = 0x10;
+constexpr int FORM_SHARE_PREV = 0x20;
+constexpr int base_form( int f ) { return f & 0xf; }
+
+// Slots and jet levels of a gather pass (SHARED: one slot read by every
+// term; otherwise one slot per term, except that a FORM_SHARE_PREV term
+// reads its predecessor's): values for value-type outputs, gradients for
+// gradient-type outputs and value-type position cotangents, Hessians for
+// gradient-type position cotangents.
 template <int... FORMS>
 struct GatherNeeds
 {
     static constexpr int N = sizeof...( FORMS );
-    static constexpr int f[N] = { FORMS... };
+    static constexpr int raw[N] = { FORMS... };
+    static constexpr int f[N] = { base_form( FORMS )... };
+    static constexpr bool xonly( int t ) { return ( raw[t] & FORM_XONLY ) != 0; }
+    static constexpr int slot_of( int t, bool shared )
+    {
+        if ( shared )
+            return 0;
+        int s = -1;
+        for ( int i = 0; i <= t; ++i )
+            if ( !( raw[i] & FORM_SHARE_PREV ) )
+                ++s;
+        return s;
+    }
+    static constexpr int nslot( bool shared ) { return slot_of( N - 1, shared ) + 1; }
+    static constexpr int first_term( int slot, bool shared )
+    {
+        for ( int t = 0; t < N; ++t )
+            if ( slot_of( t, shared ) == slot )
+                return t;
+        return 0;
+    }
     static constexpr bool uses( int t, int slot, bool shared )
     {
-        return shared || t == slot;
+        return slot_of( t, shared ) == slot;
     }
+    // The term writes its particle output / adds its position cotangent.
+    static constexpr bool out( int t, int mode ) { return mode_out( mode ) && !xonly( t ); }
+    static constexpr bool xbar( int mode ) { return mode_xbar( mode ); }
     static constexpr bool V( int slot, bool shared, int mode )
     {
         for ( int t = 0; t < N; ++t )
-            if ( uses( t, slot, shared ) && form_value_type( f[t] ) && mode_out( mode ) )
+            if ( uses( t, slot, shared ) && form_value_type( f[t] ) && out( t, mode ) )
                 return true;
         return false;
     }
@@ -692,22 +725,39 @@ struct GatherNeeds
     {
         for ( int t = 0; t < N; ++t )
             if ( uses( t, slot, shared ) &&
-                 ( ( !form_value_type( f[t] ) && mode_out( mode ) ) ||
-                   ( form_value_type( f[t] ) && mode_xbar( mode ) ) ) )
+                 ( ( !form_value_type( f[t] ) && out( t, mode ) ) ||
+                   ( form_value_type( f[t] ) && xbar( mode ) ) ) )
                 return true;
         return false;
     }
     static constexpr bool H( int slot, bool shared, int mode )
     {
         for ( int t = 0; t < N; ++t )
-            if ( uses( t, slot, shared ) && !form_value_type( f[t] ) && mode_xbar( mode ) )
+            if ( uses( t, slot, shared ) && !form_value_type( f[t] ) && xbar( mode ) )
                 return true;
         return false;
     }
+    // Grid dofs of a slot: the most any of its terms reads.
     static constexpr int NG( int slot, bool shared )
     {
-        return shared ? 3
-                      : ( ( f[slot] == FORM_SV || f[slot] == FORM_VD ) ? 1 : 3 );
+        int ng = 1;
+        for ( int t = 0; t < N; ++t )
+            if ( uses( t, slot, shared ) && !( f[t] == FORM_SV || f[t] == FORM_VD ) )
+                ng = 3;
+        return ng;
+    }
+    // Passes of more than three terms read each term's particle inputs at
+    // the start of its slot (their latency hidden by that slot's jets)
+    // instead of one particle ahead, which would hold every term's inputs
+    // for two particles in registers.
+    static constexpr bool late() { return N > 3; }
+    // Position-only terms need a pass with the position cotangent.
+    static constexpr bool valid( int mode )
+    {
+        for ( int t = 0; t < N; ++t )
+            if ( xonly( t ) && !mode_xbar( mode ) )
+                return false;
+        return true;
     }
 };
 
@@ -728,21 +778,23 @@ __device__ __forceinline__ void gather_load( const GatherArgs& a, int64_t p,
 {
     using GN = GatherNeeds<FORMS...>;
     constexpr int NT = GN::N;
-    static_for<NT>( [&]( auto I ) {
-        constexpr int t = I;
-        constexpr int NP = FormTraits<GN::f[t]>::NP;
-        if constexpr ( mode_out( MODE ) )
-            a.t[t].pout.template load<Real, NP, CONTIG>( p, in.old[t] );
-        if constexpr ( mode_xbar( MODE ) )
-            a.t[t].px.template load<Real, NP, CONTIG>( p, in.P[t] );
-    } );
-    if constexpr ( mode_xbar( MODE ) )
+    if constexpr ( !GN::late() )
+        static_for<NT>( [&]( auto I ) {
+            constexpr int t = I;
+            constexpr int NP = FormTraits<GN::f[t]>::NP;
+            if constexpr ( GN::out( t, MODE ) )
+                a.t[t].pout.template load<Real, NP, CONTIG>( p, in.old[t] );
+            if constexpr ( mode_xbar( MODE ) )
+                a.t[t].px.template load<Real, NP, CONTIG>( p, in.P[t] );
+        } );
+    if constexpr ( mode_xbar( MODE ) && !GN::late() )
         a.xbar.load<double, 3, CONTIG>( p, in.xb );
 }
 
 // One particle of a gather pass, from its global reads `in`: for each grid
 // slot, its jets, then for each term reading that slot
-//   pout = old + mult * dB/dP      (MODE has outputs)
+//   pout = old + mult * dB/dP      (MODE has outputs, the term is not
+//                                   position-only)
 //   xbar = old + mult * dB/dx      (MODE has the position cotangent).
 // Slots are finished one after the other, so only one slot's jets are live.
 // Load(slot_constant, i, j, k, f) fills the slot's NG values of stencil node
@@ -753,12 +805,14 @@ __device__ __forceinline__ void gather_compute(
     Load&& load, const GatherIn<Real, MODE, FORMS...>& in )
 {
     using GN = GatherNeeds<FORMS...>;
+    static_assert( GN::valid( MODE ), "position-only terms need the position cotangent" );
     constexpr int NT = GN::N;
-    constexpr int NSLOT = SHARED ? 1 : NT;
+    constexpr int NSLOT = GN::nslot( SHARED );
     constexpr bool XBAR = mode_xbar( MODE );
-    constexpr bool OUT = mode_out( MODE );
     double xb[3] = { 0.0, 0.0, 0.0 };
-    if constexpr ( XBAR )
+    if constexpr ( XBAR && GN::late() )
+        a.xbar.load<double, 3, CONTIG>( p, xb );
+    else if constexpr ( XBAR )
     {
 #pragma unroll
         for ( int e = 0; e < 3; ++e )
@@ -767,30 +821,47 @@ __device__ __forceinline__ void gather_compute(
     static_for<NSLOT>( [&]( auto S ) {
         constexpr int sl = S;
         constexpr int NG = GN::NG( sl, SHARED );
+        // Late inputs of this slot's terms (issued before the jets).
+        Real lold[NT][9];
+        Real lP[NT][9];
+        if constexpr ( GN::late() )
+            static_for<NT>( [&]( auto I ) {
+                constexpr int t = I;
+                if constexpr ( GN::uses( t, sl, SHARED ) )
+                {
+                    constexpr int NP = FormTraits<GN::f[t]>::NP;
+                    if constexpr ( GN::out( t, MODE ) )
+                        a.t[t].pout.template load<Real, NP, CONTIG>( p, lold[t] );
+                    if constexpr ( XBAR )
+                        a.t[t].px.template load<Real, NP, CONTIG>( p, lP[t] );
+                }
+            } );
         Jet<Real> J[3];
         eval_jets<ORDER, NG, GN::V( sl, SHARED, MODE ), GN::G( sl, SHARED, MODE ),
                   GN::H( sl, SHARED, MODE )>(
             s, [&]( int i, int j, int k, Real* fv ) { load( S, i, j, k, fv ); }, J );
         static_for<NT>( [&]( auto I ) {
             constexpr int t = I;
-            if constexpr ( SHARED || t == sl )
+            if constexpr ( GN::uses( t, sl, SHARED ) )
             {
                 constexpr int F = GN::f[t];
                 constexpr int NP = FormTraits<F>::NP;
                 const Real m = (Real)a.t[t].mult;
-                if constexpr ( OUT )
+                const Real* old_t = GN::late() ? lold[t] : in.old[t];
+                const Real* P_t = GN::late() ? lP[t] : in.P[t];
+                if constexpr ( GN::out( t, MODE ) )
                 {
                     Real out[NP];
                     jet_dP<F>( J, out );
 #pragma unroll
                     for ( int c = 0; c < NP; ++c )
-                        out[c] = in.old[t][c] + m * out[c];
+                        out[c] = old_t[c] + m * out[c];
                     a.t[t].pout.template store<Real, NP, CONTIG>( p, out );
                 }
                 if constexpr ( XBAR )
                 {
                     Real xt[3] = { Real( 0 ), Real( 0 ), Real( 0 ) };
-                    jet_dx<F>( in.P[t], J, xt );
+                    jet_dx<F>( P_t, J, xt );
 #pragma unroll
                     for ( int e = 0; e < 3; ++e )
                         xb[e] += (double)( m * xt[e] );
@@ -826,9 +897,10 @@ struct GlobalGatherLoad
     __device__ __forceinline__ void operator()( IC, int i, int j, int k,
                                                 Real* f ) const
     {
+        using GN = GatherNeeds<FORMS...>;
         constexpr int sl = IC::value;
-        constexpr int NG = GatherNeeds<FORMS...>::NG( sl, SHARED );
-        const GFieldDev& g = a.t[sl].grid;
+        constexpr int NG = GN::NG( sl, SHARED );
+        const GFieldDev& g = a.t[GN::first_term( sl, SHARED )].grid;
         const Real* q = gptr<const Real>( g, ( base[0] + i ) * g.s[0] + ( base[1] + j ) * g.s[1] +
                         ( base[2] + k ) * g.s[2] );
 #pragma unroll

@@ -226,7 +226,8 @@ inline int form_gdof( int form )
     return n[form];
 }
 
-constexpr int MAX_TERMS = 3;       // terms fused into one kernel pass
+constexpr int MAX_TERMS = 3;       // terms fused into one scatter pass
+constexpr int MAX_GATHER_TERMS = 5; // terms fused into one gather pass
 constexpr int MAX_CALL_TERMS = 16; // terms accepted by one API call
 
 // One term of a scatter pass (P2G forward, or G2P adjoint):
@@ -277,7 +278,7 @@ struct GatherArgs
     int32_t nterms;
     int32_t has_xbar;
     int32_t contig; // every particle field of the pass has vector_length 1
-    GatherTermDev t[MAX_TERMS];
+    GatherTermDev t[MAX_GATHER_TERMS];
     unsigned long long* skipped;
 };
 
@@ -395,7 +396,10 @@ enum Plan : int32_t
     PLAN_SINGLE = 0,  // one term, any form
     PLAN_MPM_P2G = 1, // (SV, VV, TD): mass, momentum, stress divergence
     PLAN_MPM_G2P = 2, // (VV, TDT) on one shared grid: velocity + gradient
-    PLAN_MASS_MOMENTUM = 3 // (SV, VV): mass and momentum (P2G only)
+    PLAN_MASS_MOMENTUM = 3, // (SV, VV): mass and momentum (P2G only)
+    // Gather only: the P2G adjoint of (SV, VV, TD) with the position part
+    // of the G2P adjoint of (VV, TDT) on one shared grid, in one pass.
+    PLAN_MPM_ADJOINT = 4
 };
 
 // Kernel launchers (return hipSuccess or the launch error), one set per

@@ -401,8 +401,10 @@ hipError_t gather_typed( Launcher& L, int plan, const GatherArgs& a,
     hipError_t e = hipSuccess;
     const int mode = gather_mode( a );
     const char* name =
-        a.has_xbar ? ( plan == PLAN_MPM_P2G ? "p2g_adjoint_mpm" : "p2g_adjoint" )
-                   : ( plan == PLAN_MPM_G2P ? "g2p_mpm" : "g2p" );
+        plan == PLAN_MPM_ADJOINT
+            ? "adjoint_mpm"
+            : a.has_xbar ? ( plan == PLAN_MPM_P2G ? "p2g_adjoint_mpm" : "p2g_adjoint" )
+                         : ( plan == PLAN_MPM_G2P ? "g2p_mpm" : "g2p" );
     L.before( name, s );
     if ( plan == PLAN_SINGLE )
     {
@@ -428,6 +430,14 @@ hipError_t gather_typed( Launcher& L, int plan, const GatherArgs& a,
     {
         hipLaunchKernelGGL(
             ( gather_kernel<Real, GATHER_OUT_XBAR, false, FORM_SV, FORM_VV, FORM_TD> ),
+            g, dim3( BLOCK ), 0, s, a );
+        e = hipGetLastError();
+    }
+    else if ( plan == PLAN_MPM_ADJOINT && mode == GATHER_OUT_XBAR )
+    {
+        hipLaunchKernelGGL(
+            ( gather_kernel<Real, GATHER_OUT_XBAR, false, FORM_SV, FORM_VV, FORM_TD,
+                            FORM_VV | FORM_XONLY, FORM_TDT | FORM_XONLY | FORM_SHARE_PREV> ),
             g, dim3( BLOCK ), 0, s, a );
         e = hipGetLastError();
     }

@@ -220,29 +220,32 @@ __device__ __forceinline__ void window_coords( const TileArgs& ta, int f, const 
 // Tiled gather
 // ---------------------------------------------------------------------------
 
+// Waves per SIMD the register allocation of a gather must leave room for:
+// three for the FP32 passes that read their inputs late (more than three
+// terms; their LDS windows allow four), no bound otherwise.
+template <class Real, int... FORMS>
+constexpr int gather_min_waves()
+{
+    return ( sizeof( Real ) == 4 && GatherNeeds<FORMS...>::late() ) ? 3 : 1;
+}
+
 template <class Real, int MODE, bool CONTIG, bool SHARED, int... FORMS>
-__global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
+__global__ __launch_bounds__( BLOCK, ( gather_min_waves<Real, FORMS...>() ) ) void gather_tiled_kernel(
     GatherArgs a, TileArgs ta )
 {
-    using FL = FormList<FORMS...>;
-    constexpr int NT = FL::N;
-    constexpr int NSLOT = SHARED ? 1 : NT;
+    using GN = GatherNeeds<FORMS...>;
+    constexpr int NSLOT = GN::nslot( SHARED );
     extern __shared__ __align__( 16 ) unsigned char smem[];
 
-    // LDS: staged windows, then the run table (capacities of a full brick).
-    Real* win[NT];
+    // LDS: one staged window per grid slot (3-dof slots padded to 4), then
+    // the run table (capacities of a full brick).
+    Real* win[NSLOT];
     size_t off = 0;
-    static_for<NT>( [&]( auto I ) {
-        constexpr int t = I;
-        if ( t < NSLOT )
-        {
-            win[t] = reinterpret_cast<Real*>( smem + off );
-            off = align16( off + sizeof( Real ) * ta.max_win_nodes * ngp_of( FL::v[t] ) );
-        }
-        else
-        {
-            win[t] = win[0];
-        }
+    static_for<NSLOT>( [&]( auto I ) {
+        constexpr int sl = I;
+        win[sl] = reinterpret_cast<Real*>( smem + off );
+        off = align16( off + sizeof( Real ) * ta.max_win_nodes *
+                                 ( GN::NG( sl, SHARED ) == 1 ? 1 : 4 ) );
     } );
     int64_t* run_start = reinterpret_cast<int64_t*>( smem + off );
     int* run_pre = reinterpret_cast<int*>( run_start + ta.max_runs );
@@ -267,11 +270,11 @@ __global__ __launch_bounds__( BLOCK ) void gather_tiled_kernel(
 
         // Stage the windows (nodes outside the grid read as zero).
         static_for<NSLOT>( [&]( auto I ) {
-            constexpr int t = I;
-            constexpr int NG = ng_of( FL::v[t] );
-            constexpr int NGP = ngp_of( FL::v[t] );
-            const GFieldDev& g = a.t[t].grid;
-            Real* wp = win[t];
+            constexpr int sl = I;
+            constexpr int NG = GN::NG( sl, SHARED );
+            constexpr int NGP = NG == 1 ? 1 : 4;
+            const GFieldDev& g = a.t[GN::first_term( sl, SHARED )].grid;
+            Real* wp = win[sl];
             for ( int f = threadIdx.x; f < wnodes * NG; f += BLOCK )
             {
                 int q[3], d;
@@ -1277,6 +1280,12 @@ hipError_t gather_tiled_launch( int plan, int mode, const GatherArgs& a, const T
         return launch_tile_kernel(
             gather_tiled_kernel<Real, GATHER_OUT_XBAR, CONTIG, false, FORM_SV, FORM_VV, FORM_TD>,
             a, t, lds, s );
+    if ( plan == PLAN_MPM_ADJOINT && mode == GATHER_OUT_XBAR )
+        return launch_tile_kernel(
+            gather_tiled_kernel<Real, GATHER_OUT_XBAR, CONTIG, false, FORM_SV, FORM_VV, FORM_TD,
+                                FORM_VV | FORM_XONLY,
+                                FORM_TDT | FORM_XONLY | FORM_SHARE_PREV>,
+            a, t, lds, s );
     return hipErrorInvalidValue;
 }
 
@@ -1286,8 +1295,10 @@ hipError_t gather_tiled_typed( Launcher& L, int plan, const GatherArgs& a,
 {
     const int mode = !a.has_xbar ? GATHER_OUT : ( a.t[0].pout.data ? GATHER_OUT_XBAR : GATHER_XBAR );
     const char* name =
-        a.has_xbar ? ( plan == PLAN_MPM_P2G ? "p2g_adjoint_mpm_tiled" : "gather_xbar_tiled" )
-                   : ( plan == PLAN_MPM_G2P ? "g2p_mpm_tiled" : "g2p_tiled" );
+        plan == PLAN_MPM_ADJOINT
+            ? "adjoint_mpm_tiled"
+            : a.has_xbar ? ( plan == PLAN_MPM_P2G ? "p2g_adjoint_mpm_tiled" : "gather_xbar_tiled" )
+                         : ( plan == PLAN_MPM_G2P ? "g2p_mpm_tiled" : "g2p_tiled" );
     L.before( name, s );
     const hipError_t e = a.contig ? gather_tiled_launch<Real, true>( plan, mode, a, t, lds, s )
                                   : gather_tiled_launch<Real, false>( plan, mode, a, t, lds, s );

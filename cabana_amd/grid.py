@@ -796,6 +796,51 @@ def g2p_adjoint(array, halo, points, num_point, spline, functor, values_bar, gri
         ctypes.byref(b) if b is not None else None, N.current_stream()), ctx)
 
 
+def adjoint_gather(p2g_functor, array, halo, points, num_point, spline, grid_bar,
+                   p2g_values_bar, g2p_functor, g2p_values_bar, points_bar, bins=None):
+    """The gather-shaped parts of both adjoints of a transfer pair in one
+    call: p2g_adjoint(p2g_functor, ..., grid_bar, p2g_values_bar, points_bar)
+    and the points_bar part of g2p_adjoint(array, ..., g2p_functor,
+    g2p_values_bar, grid_bar=None, points_bar).  ``array`` is the G2P
+    forward grid (or one per G2P functor).  For the MPM set (P2G ScalarValue,
+    VectorValue, TensorDivergence; G2P VectorValue, VectorGradient of one
+    grid) it is one pass over the particles."""
+    pf = p2g_functor if isinstance(p2g_functor, (list, tuple)) else [p2g_functor]
+    gbars = grid_bar if isinstance(grid_bar, (list, tuple)) else [grid_bar]
+    pvb = p2g_values_bar if isinstance(p2g_values_bar, (list, tuple)) else [p2g_values_bar]
+    gf = g2p_functor if isinstance(g2p_functor, (list, tuple)) else [g2p_functor]
+    arrays = array if isinstance(array, (list, tuple)) else [array] * len(gf)
+    gvb = g2p_values_bar if isinstance(g2p_values_bar, (list, tuple)) else [g2p_values_bar]
+    if len(gbars) != len(pf) or len(pvb) != len(pf) or len(arrays) != len(gf) or \
+            len(gvb) != len(gf):
+        raise ValueError("one grid_bar and values_bar per P2G functor, one array and "
+                         "values_bar per G2P functor")
+    m = _call_mesh(list(gbars) + list(arrays))
+    pts = _points_descriptor(points, num_point)
+    pt = (N.P2GAdjointTerm * len(pf))()
+    for i, f in enumerate(pf):
+        pt[i].op = f.op
+        pt[i].multiplier = f._multiplier
+        pt[i].values = _values_descriptor(f._x, num_point)
+        pt[i].grid_bar = gbars[i].descriptor()
+        pt[i].values_bar = particle_descriptor(pvb[i]) if pvb[i] is not None \
+            else N.ParticleField()
+    gt = (N.G2PAdjointTerm * len(gf))()
+    for i, f in enumerate(gf):
+        gt[i].op = f.op
+        gt[i].multiplier = f._multiplier
+        gt[i].grid = arrays[i].descriptor()
+        gt[i].values_bar = _values_descriptor(gvb[i], num_point)
+        gt[i].grid_bar = N.GridField()
+    xb = particle_descriptor(points_bar)
+    b = _bins_descriptor(bins)
+    ctx = N.context()
+    N.check(N.load().cabana_amd_adjoint_gather(
+        ctx, ctypes.byref(m), spline.order, ctypes.byref(pts), len(pf), pt, len(gf), gt,
+        _halo_width(halo), ctypes.byref(xb), ctypes.byref(b) if b is not None else None,
+        N.current_stream()), ctx)
+
+
 def check_domain():
     """Waits for the stream and raises if particles left the ghosted grid."""
     ctx = N.context()

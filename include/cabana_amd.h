@@ -354,6 +354,26 @@ cabana_amd_status cabana_amd_g2p_adjoint(
     int32_t halo_width, const cabana_amd_particle_field* points_bar,
     const cabana_amd_binning* bins, cabana_amd_stream stream );
 
+/* The gather-shaped parts of both adjoints of a transfer pair in one call:
+ * cabana_amd_p2g_adjoint of p2g_terms, and the position part of
+ * cabana_amd_g2p_adjoint of g2p_terms (their grid_bar is neither read nor
+ * written; run cabana_amd_g2p_adjoint with points_bar NULL for the grid
+ * part, before or after).  points_bar (required) += both position
+ * cotangents; the result equals cabana_amd_p2g_adjoint followed by
+ * cabana_amd_g2p_adjoint with grid_bar NULL, up to the order of the
+ * additions into points_bar.  For the MPM set (P2G ScalarValue,
+ * VectorValue, TensorDivergence; G2P VectorValue, VectorGradient of one
+ * grid that is none of the grid_bar) it is one pass over the particles,
+ * which reads each position and position cotangent once; other sets run
+ * the passes of the two calls. */
+cabana_amd_status cabana_amd_adjoint_gather(
+    cabana_amd_context* ctx, const cabana_amd_mesh* mesh, int32_t spline_order,
+    const cabana_amd_particle_field* points, int32_t num_p2g_terms,
+    const cabana_amd_p2g_adjoint_term* p2g_terms, int32_t num_g2p_terms,
+    const cabana_amd_g2p_adjoint_term* g2p_terms, int32_t halo_width,
+    const cabana_amd_particle_field* points_bar, const cabana_amd_binning* bins,
+    cabana_amd_stream stream );
+
 /* ----------------------------------------------------------------------- */
 /* Halo (single rank, periodic dimensions exchange with themselves)          */
 /* ----------------------------------------------------------------------- */
