@@ -735,9 +735,10 @@ size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng, int nslot,
 // Brick shape and part capacity.  Gathers use the largest cubic brick whose
 // windows fit three workgroups per CU, with parts of ~1.25x the mean brick
 // population.  Scatters sort a whole part in LDS: the brick is the largest
-// near-cubic one whose expected population (x1.2) fits a part that, with the
-// FP64 window, keeps three workgroups per CU for FP32 records and two for
-// FP64 ones (the kernel's register budget); denser bricks are cut into
+// near-cubic one whose expected population (x1.2) fits a part (at most two
+// particles per thread) that, with the FP64 window, stays within 52 KB of
+// LDS for FP32 records and 78 KB for FP64 ones (two workgroups per CU run
+// at once, the kernel's register budget); denser bricks are cut into
 // several parts.
 bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int rec_len,
                    int order, double mean_ppc, TileShape& out )

@@ -829,9 +829,9 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
 }
 
 // Workgroup size of the scatter and its minimum waves per SIMD (the
-// register budget): three 256-thread workgroups per CU for FP32 records,
-// two for FP64 ones (whose accumulators and records take twice the
-// registers).
+// register budget): two 256-thread workgroups per CU.  For FP32 records a
+// bound of three fits the LDS too, but its 168 registers make the MPM pass
+// spill (it needs about 190), which measured 8 % slower.
 template <class Real>
 constexpr int sblock()
 {
@@ -840,7 +840,7 @@ constexpr int sblock()
 template <class Real>
 constexpr int scatter_min_waves()
 {
-    return sizeof( Real ) == 4 ? 3 : 2;
+    return 2;
 }
 
 template <class Real, bool CONTIG, bool SHARED, int... FORMS>
@@ -1119,16 +1119,6 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
             task[len_off[sl >> 16] + ( sl & 0xffff )] = task_in[q];
         }
         __syncthreads();
-        // The next part's particle reads are issued here and complete while
-        // this part accumulates.
-        if ( more )
-        {
-            const int nruns1 = ( tg1.c1[0] - tg1.c0[0] ) * ( tg1.c1[1] - tg1.c0[1] );
-            const int pbeg1 = tg1.part * ta.cap;
-            prefetch_part<Real, CONTIG, MAXM, SBLOCK, FORMS...>(
-                a, ta, run_start[buf ^ 1], run_pre[buf ^ 1], nruns1, pbeg1,
-                min( ta.cap, run_pre[buf ^ 1][nruns1] - pbeg1 ), pf );
-        }
         // D: a task's K*K lanes each own one (j, k) row of the stencil: they
         // walk the task's records (the same record across the group: an
         // LDS broadcast), accumulate the K nodes of their row in registers
@@ -1168,6 +1158,17 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
                     for ( int d = 0; d < DT; ++d )
                         atomicAdd( wb + i * wstride_i + d, (double)acc[i][d] );
             }
+        }
+        // The next part's particle reads are issued here, after the
+        // accumulation (whose registers they would otherwise share), and
+        // complete while this part's window is flushed.
+        if ( more )
+        {
+            const int nruns1 = ( tg1.c1[0] - tg1.c0[0] ) * ( tg1.c1[1] - tg1.c0[1] );
+            const int pbeg1 = tg1.part * ta.cap;
+            prefetch_part<Real, CONTIG, MAXM, SBLOCK, FORMS...>(
+                a, ta, run_start[buf ^ 1], run_pre[buf ^ 1], nruns1, pbeg1,
+                min( ta.cap, run_pre[buf ^ 1][nruns1] - pbeg1 ), pf );
         }
         __syncthreads();
 
