@@ -736,10 +736,9 @@ size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng, int nslot,
 // windows fit three workgroups per CU, with parts of ~1.25x the mean brick
 // population.  Scatters sort a whole part in LDS: the brick is the largest
 // near-cubic one whose expected population (x1.2) fits a part (at most two
-// particles per thread) that, with the FP64 window, stays within 52 KB of
-// LDS for FP32 records and 78 KB for FP64 ones (two workgroups per CU run
-// at once, the kernel's register budget); denser bricks are cut into
-// several parts.
+// particles per thread) that, with the FP64 window, stays within 78 KB of
+// LDS (two workgroups per CU run at once, the kernel's register budget);
+// denser bricks are cut into several parts.
 bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int rec_len,
                    int order, double mean_ppc, TileShape& out )
 {
@@ -762,7 +761,7 @@ bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int 
         }
         return false;
     }
-    const size_t budget = fp64 ? 78 * 1024 : 52 * 1024;
+    const size_t budget = 78 * 1024;
     auto with_cap = [&]( TileShape sh, int cap ) {
         sh.cap = cap;
         sh.max_tasks = base_cells( sh.T, order ) + cap / SCATTER_TASK_MAX + 1;
@@ -1027,7 +1026,7 @@ cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
     a.contig = pf_contig( a.points ) && pf_contig( a.xbar );
     for ( int i = 0; i < a.nterms; ++i )
         a.contig = a.contig && pf_contig( a.t[i].pin );
-    int rec_len = 3;
+    int rec_len = 2 * ( order + 1 ) + 2;
     for ( int i = 0; i < a.nterms; ++i )
         rec_len += form_pcomp( forms[i] );
     TileArgs t;

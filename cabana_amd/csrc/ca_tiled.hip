@@ -50,6 +50,9 @@ constexpr int BLOCK = 256;
 // c + halo + BASE_SHIFT + {0, 1 if ORDER == 2}.
 constexpr int BASE_SHIFT = ( ORDER == 1 ) ? 0 : -1;
 constexpr int BASE_SPREAD = ( ORDER == 2 ) ? 1 : 0;
+// Scatter record header: the x weights and weight gradients of the K nodes,
+// then the y and z fractional offsets.
+constexpr int RHEAD = 2 * K + 2;
 
 template <int... FORMS>
 struct FormList
@@ -510,15 +513,16 @@ __device__ __forceinline__ float fma_r( float a, float b, float c ) { return fma
 __device__ __forceinline__ double fma_r( double a, double b, double c ) { return fma( a, b, c ); }
 
 
-// Record layout of a scatter pass: 3 fractional offsets, then the
-// attributes of every term in order.
+// Record layout of a scatter pass: the RHEAD header (x weights and weight
+// gradients, y and z fractional offsets), then the attributes of every
+// term in order.
 template <int... FORMS>
 struct RecordLayout
 {
     using FL = FormList<FORMS...>;
     static constexpr int attr( int t )
     {
-        int o = 3;
+        int o = RHEAD;
         for ( int i = 0; i < t; ++i )
             o += np_of( FL::v[i] );
         return o;
@@ -721,10 +725,12 @@ struct PencilDofs
 };
 
 // One record's contribution to a lane's K x DT accumulator (nodes i of the
-// lane's (j, k) row).  The record holds the particle's three cell offsets
-// and its premultiplied attributes.
+// lane's (j, k) row).  The record holds the particle's x weights and weight
+// gradients (formed once when the record is written, instead of by each of
+// the task's K*K lanes), its y and z cell offsets and its premultiplied
+// attributes.
 template <class Real, bool SHARED, int... FORMS>
-__device__ __forceinline__ void pencil_record( const Real* rp, Real rdx0, const NodePoly<Real>& py,
+__device__ __forceinline__ void pencil_record( const Real* rp, const NodePoly<Real>& py,
                                                const NodePoly<Real>& pz,
                                                Real ( *acc )[SlotDofs<FORMS...>::total( SHARED )] )
 {
@@ -733,13 +739,11 @@ __device__ __forceinline__ void pencil_record( const Real* rp, Real rdx0, const 
     using RL = RecordLayout<FORMS...>;
     using PD = PencilDofs<SHARED, FORMS...>;
     constexpr int DT = SD::total( SHARED );
-    Spline1D<ORDER, Real> sx;
-    spline_weights<ORDER, Real, false>( rp[0], rdx0, sx );
-    const Real* wx = sx.w;
-    const Real* gx = sx.g;
+    const Real* wx = rp;
+    const Real* gx = rp + K;
     Real wy, gy, wz, gz;
-    py.eval( rp[1], wy, gy );
-    pz.eval( rp[2], wz, gz );
+    py.eval( rp[2 * K], wy, gy );
+    pz.eval( rp[2 * K + 1], wz, gz );
     const Real A = wy * wz, B = gy * wz, C = wy * gz;
     Real al[DT], be[DT];
 #pragma unroll
@@ -792,7 +796,7 @@ template <class Real, bool CONTIG, int MAXM, int NTH, int... FORMS>
 __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileArgs& ta,
                                                const int64_t* run_start, const int* run_pre,
                                                int nruns, int pbeg, int nbatch,
-                                               PartPrefetch<Real, MAXM, RecordLayout<FORMS...>::len() - 3>& pf )
+                                               PartPrefetch<Real, MAXM, RecordLayout<FORMS...>::len() - RHEAD>& pf )
 {
     using FL = FormList<FORMS...>;
     using RL = RecordLayout<FORMS...>;
@@ -823,7 +827,7 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
         static_for<FL::N>( [&]( auto I ) {
             constexpr int t = I;
             a.t[t].pin.template load<Real, np_of( FL::v[t] ), CONTIG>(
-                p, &pf.attr[mi][RL::attr( t ) - 3] );
+                p, &pf.attr[mi][RL::attr( t ) - RHEAD] );
         } );
     }
 }
@@ -856,7 +860,7 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
     constexpr int RS = record_stride( RLEN, (int)sizeof( Real ) );
     constexpr int SBLOCK = sblock<Real>();
     constexpr int MAXM = SCATTER_SLOTS;
-    constexpr int NPT = RLEN - 3;
+    constexpr int NPT = RLEN - RHEAD;
     constexpr int LMAX = SCATTER_TASK_MAX;
     // Pencil lanes: a task's K*K lanes own the (j, k) rows of its stencil.
     constexpr int KK = K * K;
@@ -928,7 +932,6 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
     const int pj = lg / K, pk = lg - ( lg / K ) * K;
     const bool pencil_lane = gw < GPW;
     const int gid = ( (int)threadIdx.x >> 6 ) * GPW + gw;
-    const Real rdx0 = (Real)a.mesh.rdx[0];
     NodePoly<Real> py, pz;
     py.init( pj, (Real)a.mesh.rdx[1] );
     pz.init( pk, (Real)a.mesh.rdx[2] );
@@ -982,9 +985,9 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
 
         // A: every particle of the part (loaded into registers during the
         // previous part) gets its stencil base cell, counted per base cell
-        // (the slot is the particle's rank within it), and its record (cell
-        // offsets, premultiplied attributes) is written to LDS at its part
-        // index.
+        // (the slot is the particle's rank within it), and its record (x
+        // weights and gradients, y and z cell offsets, premultiplied
+        // attributes) is written to LDS at its part index.
         int key[MAXM];
 #pragma unroll
         for ( int mi = 0; mi < MAXM; ++mi )
@@ -997,13 +1000,26 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
             // else base cell << 16 | rank.
             int bs[3];
             Real v[RLEN];
+            Real fr[3];
             bool ok = true;
 #pragma unroll
             for ( int d = 0; d < 3; ++d )
                 ok = spline_frac<ORDER, Real>( logical_coord<ORDER>( pf.x[mi][d], a.mesh.low[d],
                                                                      a.mesh.rdx[d] ),
-                                               a.mesh.num_node[d], bs[d], v[d] ) &&
+                                               a.mesh.num_node[d], bs[d], fr[d] ) &&
                      ok;
+            {
+                Spline1D<ORDER, Real> sx;
+                spline_weights<ORDER, Real, false>( fr[0], (Real)a.mesh.rdx[0], sx );
+#pragma unroll
+                for ( int k = 0; k < K; ++k )
+                {
+                    v[k] = sx.w[k];
+                    v[K + k] = sx.g[k];
+                }
+                v[2 * K] = fr[1];
+                v[2 * K + 1] = fr[2];
+            }
             if ( !ok )
             {
                 atomicAdd( a.skipped, 1ull );
@@ -1027,7 +1043,7 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
                     const Real m = (Real)a.t[t].mult;
 #pragma unroll
                     for ( int c = 0; c < NP; ++c )
-                        v[RL::attr( t ) + c] = pf.attr[mi][RL::attr( t ) - 3 + c] * m;
+                        v[RL::attr( t ) + c] = pf.attr[mi][RL::attr( t ) - RHEAD + c] * m;
                 } );
                 detail::vec_copy_out<Real, RLEN>( v, rec + (size_t)RS * i );
             }
@@ -1149,7 +1165,7 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
                 {
                     Real cur[RLEN];
                     read_record<Real, RLEN>( rec + (size_t)RS * ord[r], cur );
-                    pencil_record<Real, SHARED, FORMS...>( cur, rdx0, py, pz, acc );
+                    pencil_record<Real, SHARED, FORMS...>( cur, py, pz, acc );
                 }
                 double* wb = win + ( ( lx * W[1] + ly + pj ) * W[2] + lz + pk ) * DT;
 #pragma unroll
