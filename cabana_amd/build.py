@@ -96,6 +96,12 @@ def build_cpp_tests():
           src, "-o", out, "-L", HERE, "-lcabana_amd", "-L", os.path.join(rocm, "lib"),
           "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../cabana_amd",
           "-Wl,-rpath," + os.path.join(rocm, "lib")])
+    # User functors through include/cabana_amd_custom.hpp: a HIP program.
+    src = os.path.join(ROOT, "tests", "cpp", "test_custom_functor.hip")
+    custom = os.path.join(ROOT, "tests", "cpp", "test_custom_functor")
+    _run([_hipcc()] + COMMON + ["-I", os.path.join(ROOT, "include"),
+                                "-I", os.path.join(ROOT, "tests", "cpp"), src, "-o", custom,
+                                "-L", HERE, "-lcabana_amd", "-Wl,-rpath,$ORIGIN/../../cabana_amd"])
     return out
 
 

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "ca_internal.hpp"
+#include "../../include/cabana_amd_spline.hpp"
 
 #include <utility>
 
@@ -70,53 +71,6 @@ __device__ __forceinline__ bool spline_frac( double x0, int num_node, int& base,
         return false;
     f = (Real)( x0 - (double)i0 );
     return true;
-}
-
-// The 1-D B-spline weight of stencil node j as a polynomial of the
-// fractional offset f: w_j(f) = sum_n spline_coef(ORDER, j, n) t^n with
-// t = f - spline_shift(ORDER, j).  These are the weights of Spline<0> ..
-// Spline<6>::value of Cabana_Grid_Splines.hpp (:94-100, :184-190, :281-299,
-// :394-428, :522-557, :668-708, :833-881), written there as polynomials in
-// x0 - int(x0) (minus 0.5 for orders 4 to 6) and here re-expanded around the
-// end of [0, 1) where the node's weight is smallest (the middle for the
-// centre node of an even order): the edge nodes' weights, (1 - f)^N / N! and
-// f^N / N!, become a single power of t, and a small weight keeps its
-// relative accuracy (a sum of O(1) terms would leave an absolute error of
-// one rounding of 1 in FP32, enough to change a nearly empty node's mass by
-// O(1)).
-__host__ __device__ constexpr double spline_shift( int order, int j )
-{
-    return 2 * j < order ? 1.0 : ( 2 * j == order ? 0.5 : 0.0 );
-}
-
-// Coefficient of t^n in w_j.  Node j of a degree-N spline sees the
-// polynomial piece w_j(f) = 1/N! sum_{k=0}^{N-j} (-1)^k C(N+1, k)
-// (f + N - j - k)^N of the uniform B-spline (truncated-power form); with
-// f = t + s each power expands binomially.  Every partial sum is a dyadic
-// rational with fewer than 53 significant bits, so the sum is exact and
-// the one division by N! rounds correctly.  Evaluated at compile time only
-// (every caller binds it to a constexpr).
-__host__ __device__ constexpr double spline_coef( int order, int j, int n )
-{
-    const double s = spline_shift( order, j );
-    double fact = 1.0;
-    for ( int i = 2; i <= order; ++i )
-        fact *= i;
-    double binom_n = 1.0; // C(order, n)
-    for ( int i = 0; i < n; ++i )
-        binom_n = binom_n * ( order - i ) / ( i + 1 );
-    double sum = 0.0;
-    double binom_k = 1.0; // C(order + 1, k)
-    for ( int k = 0; k <= order - j; ++k )
-    {
-        const double c = s + ( order - j - k );
-        double p = 1.0;
-        for ( int e = 0; e < order - n; ++e )
-            p *= c;
-        sum += ( k % 2 ? -1.0 : 1.0 ) * binom_k * binom_n * p;
-        binom_k = binom_k * ( order + 1 - k ) / ( k + 1 );
-    }
-    return sum / fact;
 }
 
 // Value, first and second derivative (in f) of node j's weight polynomial

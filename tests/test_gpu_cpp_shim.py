@@ -44,3 +44,21 @@ def test_cpp_grid_shim_under_asan_and_ubsan():
     assert proc.returncode == 0, proc.stdout
     assert "ALL PASSED" in proc.stdout
     assert "AddressSanitizer" not in proc.stdout and "runtime error" not in proc.stdout
+
+
+CUSTOM_BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_custom_functor")
+
+
+def test_cpp_custom_functors():
+    """User-written functors through include/cabana_amd_custom.hpp (the
+    functor escape hatch of Cabana's p2g / g2p): every built-in operator
+    written again with the P2G:: / G2P:: helpers matches the library's own
+    kernels for Spline<1> to Spline<3>."""
+    if not os.path.exists(CUSTOM_BIN):
+        pytest.fail("tests/cpp/test_custom_functor is not built (run __graft_entry__.build())")
+    print("running", CUSTOM_BIN, flush=True)
+    proc = subprocess.run([CUSTOM_BIN], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                          text=True, timeout=120)
+    print(proc.stdout)
+    assert proc.returncode == 0, proc.stdout
+    assert "ALL PASSED" in proc.stdout
