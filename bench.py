@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--mode", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the differentiation and checkpointed-loop figures")
+    ap.add_argument("--lds-probe", action="store_true",
+                    help="only run the LDS atomic-add probe (cabana_amd_probe_lds_add) and "
+                         "print its cycles per wave-wide add for FP32 and FP64")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one elementwise copy (y = 2*x, FP32) of this many MiB: "
                          "a kernel of known traffic against which rocprofv3 FETCH_SIZE and "
@@ -555,6 +558,18 @@ def main():
     import cabana_amd as ca
     from cabana_amd import _native
     torch.cuda.set_device(0)
+    if args.lds_probe:
+        import ctypes
+        out = {}
+        for label, fp64 in (("ds_add_f32", 0), ("ds_add_f64", 1)):
+            c = ctypes.c_double(0.0)
+            ctx = _native.context()
+            _native.check(_native.load().cabana_amd_probe_lds_add(ctx, fp64, 4096,
+                                                                    ctypes.byref(c)), ctx)
+            out[label] = c.value
+        print(json.dumps({"lds_atomic_add_cycles_per_wave_op_per_cu": out,
+                          "probe": "16 waves per CU, 4096 adds per lane into distinct LDS words"}))
+        return
     if args.copy_probe_mib > 0:
         x = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)

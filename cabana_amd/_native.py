@@ -145,6 +145,7 @@ SIGNATURES = {
     "cabana_amd_g2p_adjoint": (c_int32, [c_void_p, P(Mesh), c_int32, P(ParticleField),
                                          c_int32, P(G2PAdjointTerm), c_int32,
                                          P(ParticleField), P(Binning), c_void_p]),
+    "cabana_amd_probe_lds_add": (c_int32, [c_void_p, c_int32, c_int32, P(c_double)]),
     "cabana_amd_adjoint_gather": (c_int32, [c_void_p, P(Mesh), c_int32, P(ParticleField),
                                             c_int32, P(P2GAdjointTerm), c_int32,
                                             P(G2PAdjointTerm), c_int32, P(ParticleField),

@@ -66,8 +66,9 @@ def build_library(verbose=False):
             obj = os.path.join(BUILD, "%s_o%d.o" % (src, order))
             jobs.append([hipcc] + COMMON + ["-DCA_ORDER=%d" % order, "-c",
                                             os.path.join(CSRC, src + ".hip"), "-o", obj])
-    jobs.append([hipcc] + COMMON + ["-c", os.path.join(CSRC, "ca_grid.hip"), "-o",
-                                    os.path.join(BUILD, "ca_grid.o")])
+    for src in ("ca_grid", "ca_probe"):
+        jobs.append([hipcc] + COMMON + ["-c", os.path.join(CSRC, src + ".hip"), "-o",
+                                        os.path.join(BUILD, src + ".o")])
     for src in ("ca_api", "ca_no_alloc"):
         jobs.append([hipcc] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src + ".cpp"),
                                         "-o", os.path.join(BUILD, src + ".o")])

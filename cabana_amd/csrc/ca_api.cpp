@@ -2665,6 +2665,30 @@ extern "C" cabana_amd_status cabana_amd_particle_migrate(
 // Profiling
 // ---------------------------------------------------------------------------
 
+extern "C" cabana_amd_status cabana_amd_probe_lds_add( cabana_amd_context* ctx, int32_t fp64,
+                                                      int32_t iters,
+                                                      double* cycles_per_wave_op )
+{
+    if ( !ctx || !cycles_per_wave_op || iters < 1 || iters > ( 1 << 24 ) )
+        return fail( ctx, CABANA_AMD_ERROR_INVALID_ARGUMENT,
+                     "need ctx, an output and 1 <= iters <= 2^24" );
+    int dev = 0, cus = 0, khz = 0;
+    CA_HIP( ctx, hipGetDevice( &dev ), "get device" );
+    CA_HIP( ctx, hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ),
+            "CU count" );
+    CA_HIP( ctx, hipDeviceGetAttribute( &khz, hipDeviceAttributeClockRate, dev ), "clock rate" );
+    const int groups = 4 * cus; // 16 waves per CU
+    void* out = nullptr;
+    CA_HIP( ctx, hipMalloc( &out, sizeof( double ) * groups ), "probe buffer" );
+    float ms = 0.0f;
+    const hipError_t e = probe_lds_add( fp64 != 0, iters, groups, out, nullptr, ms );
+    (void)hipFree( out );
+    CA_HIP( ctx, e, "LDS probe" );
+    // Each CU runs 16 waves of iters wave-wide adds.
+    *cycles_per_wave_op = (double)ms * 1e-3 * (double)khz * 1e3 / ( 16.0 * iters );
+    return CABANA_AMD_OK;
+}
+
 extern "C" cabana_amd_status cabana_amd_set_deterministic( cabana_amd_context* ctx,
                                                            int32_t enable )
 {

@@ -573,4 +573,9 @@ cabana_amd_status p2g_fixed_scratch( cabana_amd_context* ctx, const cabana_amd_m
 // Records msg as the context's (and the thread's) last error; returns st.
 cabana_amd_status fail_with( cabana_amd_context* ctx, cabana_amd_status st, const char* msg );
 
+// LDS atomic-add throughput probe (ca_probe.hip): iters adds per lane over
+// num_groups workgroups of 256 threads; ms = kernel time.
+hipError_t probe_lds_add( bool fp64, int iters, int num_groups, void* out, hipStream_t s,
+                          float& ms );
+
 } // namespace cabana_amd

@@ -501,6 +501,14 @@ cabana_amd_status cabana_amd_particle_migrate(
  * global atomics); it is on by default.  No reference counterpart. */
 cabana_amd_status cabana_amd_set_auto_binning( cabana_amd_context* ctx, int32_t enable );
 
+/* LDS atomic-add throughput probe (no reference counterpart: the
+ * measurement behind the scatter's FP64 LDS window).  Runs 16 waves per CU,
+ * each lane adding iters times into its own LDS word, with ds_add_f64
+ * (fp64 != 0) or ds_add_f32, and returns the CU cycles per wave-wide add
+ * (kernel time x clock / (16 x iters)).  Synchronous; for measurement only. */
+cabana_amd_status cabana_amd_probe_lds_add( cabana_amd_context* ctx, int32_t fp64, int32_t iters,
+                                            double* cycles_per_wave_op );
+
 /* Deterministic P2G (enable = 1; off by default).  cabana_amd_p2g and the
  * P2G half of cabana_amd_g2p2g then bin the particles themselves and run a
  * pull pass per term: every node of the grid sums the contributions of the
