@@ -216,6 +216,44 @@ def unbinned_figure(torch, ca, cells, ppc, order, steps=3, warmup=1):
     return out
 
 
+def deterministic_figure(torch, ca, cells, ppc, order, calls=3):
+    """The step's forward P2G (mass, momentum, stress divergence on
+    cell-sorted particles) with the default tiled scatter and in
+    deterministic mode (cabana_amd_set_deterministic: the library's stable
+    binning, then one pull pass per term, each node summing its particles in
+    a fixed order), FP32-mixed; whether repeated deterministic calls gave
+    bitwise equal grids."""
+    w = Workload(torch, ca, torch.float32, cells, ppc, order)
+    grids = [w.gmass, w.gmom, w.gstress]
+
+    def p2g(bins):
+        for a in grids:
+            a.view().zero_()
+        ca.p2g(w.p2g_f, w.pos, w.n, w.sp, w.halo, grids, bins=bins)
+
+    out = {}
+    for label, det in (("tiled", False), ("deterministic", True)):
+        ca.set_deterministic(det)
+        try:
+            p2g(w.bins)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                p2g(w.bins)
+            torch.cuda.synchronize()
+            out[label + "_ms"] = (time.perf_counter() - t0) / calls * 1e3
+            if det:
+                ref = [a.view().clone() for a in grids]
+                p2g(w.bins)
+                out["deterministic_bitwise_repeatable"] = all(
+                    torch.equal(a.view(), r) for a, r in zip(grids, ref))
+        finally:
+            ca.set_deterministic(False)
+    del w
+    torch.cuda.empty_cache()
+    return out
+
+
 def field_errors(torch, got, ref):
     out = {}
     for k in ref:
@@ -634,6 +672,9 @@ def main():
         extra["differentiation"] = differentiation_figure(torch, ca)
         torch.cuda.empty_cache()
         extra["unbinned"] = unbinned_figure(torch, ca, args.cells, args.ppc, args.order)
+        torch.cuda.empty_cache()
+        extra["deterministic_p2g"] = deterministic_figure(torch, ca, args.cells, args.ppc,
+                                                          args.order)
         torch.cuda.empty_cache()
         extra["checkpointed_loop"] = loop_figure(torch, ca)
         torch.cuda.empty_cache()

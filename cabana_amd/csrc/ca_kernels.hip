@@ -590,28 +590,30 @@ __global__ __launch_bounds__( BLOCK ) void p2g_pull_kernel( PullArgs a )
                     const int64_t p = a.perm[q];
                     double px[3];
                     load_position( a.points, p, px );
-                    Real w[3], gr[3];
+                    // Stencil test first (most candidates miss the node),
+                    // weights only for the particles that reach it.
+                    Real f[3];
+                    int j[3];
                     bool hit = true;
 #pragma unroll
                     for ( int d = 0; d < 3; ++d )
                     {
                         const double x0 = logical_coord<ORDER>( px[d], m.low[d], m.rdx[d] );
-                        int base;
-                        Real f;
-                        Spline1D<ORDER, Real> s1;
-                        if ( !spline_frac<ORDER, Real>( x0, m.num_node[d], base, f ) )
-                        {
-                            hit = false;
-                            w[d] = gr[d] = Real( 0 );
-                            continue;
-                        }
-                        const int j = nd[d] - base;
-                        hit = hit && j >= 0 && j < K;
-                        spline_weights<ORDER, Real, false>( f, (Real)m.rdx[d], s1 );
-                        pick_node( s1, j, w[d], gr[d] );
+                        int base = 0;
+                        const bool in = spline_frac<ORDER, Real>( x0, m.num_node[d], base, f[d] );
+                        j[d] = nd[d] - base;
+                        hit = hit && in && j[d] >= 0 && j[d] < K;
                     }
                     if ( !hit )
                         continue;
+                    Real w[3], gr[3];
+#pragma unroll
+                    for ( int d = 0; d < 3; ++d )
+                    {
+                        Spline1D<ORDER, Real> s1;
+                        spline_weights<ORDER, Real, false>( f[d], (Real)m.rdx[d], s1 );
+                        pick_node( s1, j[d], w[d], gr[d] );
+                    }
                     NodeW<Real, false> nw;
                     const Real wyz = w[1] * w[2];
                     nw.W = w[0] * wyz;
