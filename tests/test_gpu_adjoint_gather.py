@@ -175,3 +175,38 @@ def test_points_bar_is_required(ca, oracle):
     fs, gs, ub, vb = pr.terms()
     with pytest.raises(CabanaAmdError):
         ca.adjoint_gather(fs, u, c.halo(u), c.pts, c.n, ca.Spline(2), gbars, vb, gs, ub, None)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_mpm_set_with_aosoa_fields(ca, oracle, order, dtype):
+    """Every particle field of the fused pass (values, value cotangents and
+    G2P output cotangents) as members of one AoSoA with VL = 16: the
+    interleaved-field form of the pass, which reads its inputs per slot."""
+    pr = Problem(ca, oracle, order, dtype, True, seed=70 + order, n=6000)
+    c = pr.c
+    vl = 16
+    nsoa = (c.n + vl - 1) // vl
+    layout = {"m": (0, ()), "q": (1, (3,)), "s": (4, (3, 3)), "mb": (13, ()),
+              "qb": (14, (3,)), "sb": (17, (3, 3)), "uv": (26, (3,)), "ul": (29, (3, 3))}
+    stride = vl * 38
+    raw = torch.zeros(nsoa * stride, dtype=dtype, device="cuda")
+    sl = {k: ca.AoSoASlice(raw[vl * o:], c.n, shape, vl, stride) for k, (o, shape) in layout.items()}
+    t = lambda a: torch.tensor(a, dtype=dtype, device="cuda")  # noqa: E731
+    sl["m"].from_tensor(t(pr.m[:, 0]))
+    sl["q"].from_tensor(t(pr.q))
+    sl["s"].from_tensor(t(pr.sig.reshape(-1, 3, 3)))
+    sl["uv"].from_tensor(t(pr.ubar_v))
+    sl["ul"].from_tensor(t(pr.ubar_l.reshape(-1, 3, 3)))
+    gbars, u = pr.arrays()
+    fs = [ca.createScalarValueP2G(sl["m"], 1.0), ca.createVectorValueP2G(sl["q"], 1.0),
+          ca.createTensorDivergenceP2G(sl["s"], -0.25)]
+    gs = [ca.createVectorValueG2P(sl["uv"], 0.7), ca.createVectorGradientG2P(sl["ul"], -1.1)]
+    xbar = torch.tensor(pr.xbar0, device="cuda")
+    ca.adjoint_gather(fs, u, c.halo(u), c.pts, c.n, ca.Spline(order), gbars,
+                      [sl["mb"], sl["qb"], sl["sb"]], gs, [sl["uv"], sl["ul"]], xbar, bins=pr.bins)
+    vb = [sl[k].to_tensor().reshape(c.n, -1).double().cpu().numpy() for k in ("mb", "qb", "sb")]
+    check(pr, oracle, (vb, xbar.cpu().numpy()))
+    # The inputs are left as they were.
+    np.testing.assert_array_equal(sl["uv"].to_tensor().cpu().numpy(),
+                                  t(pr.ubar_v).cpu().numpy())
