@@ -1026,7 +1026,9 @@ cabana_amd_status scatter_pass( cabana_amd_context* ctx, int order, bool fp64,
     a.contig = pf_contig( a.points ) && pf_contig( a.xbar );
     for ( int i = 0; i < a.nterms; ++i )
         a.contig = a.contig && pf_contig( a.t[i].pin );
-    int rec_len = 2 * ( order + 1 ) + 2;
+    // Record header of the cell-tiled scatter: the x weights and gradients
+    // with the y and z offsets for FP32 records, the three offsets for FP64.
+    int rec_len = fp64 ? 3 : 2 * ( order + 1 ) + 2;
     for ( int i = 0; i < a.nterms; ++i )
         rec_len += form_pcomp( forms[i] );
     TileArgs t;

@@ -50,9 +50,21 @@ constexpr int BLOCK = 256;
 // c + halo + BASE_SHIFT + {0, 1 if ORDER == 2}.
 constexpr int BASE_SHIFT = ( ORDER == 1 ) ? 0 : -1;
 constexpr int BASE_SPREAD = ( ORDER == 2 ) ? 1 : 0;
-// Scatter record header: the x weights and weight gradients of the K nodes,
-// then the y and z fractional offsets.
-constexpr int RHEAD = 2 * K + 2;
+// Scatter record header.  FP32 records carry the x weights and weight
+// gradients of the K nodes, then the y and z fractional offsets (the x
+// weights are formed once per particle instead of by each of a task's K*K
+// lanes); FP64 records, whose header would double in bytes, carry the
+// three fractional offsets.
+template <class Real>
+constexpr bool x_weights_in_record()
+{
+    return sizeof( Real ) == 4;
+}
+template <class Real>
+constexpr int rhead()
+{
+    return x_weights_in_record<Real>() ? 2 * K + 2 : 3;
+}
 
 template <int... FORMS>
 struct FormList
@@ -513,16 +525,15 @@ __device__ __forceinline__ float fma_r( float a, float b, float c ) { return fma
 __device__ __forceinline__ double fma_r( double a, double b, double c ) { return fma( a, b, c ); }
 
 
-// Record layout of a scatter pass: the RHEAD header (x weights and weight
-// gradients, y and z fractional offsets), then the attributes of every
-// term in order.
-template <int... FORMS>
+// Record layout of a scatter pass: the rhead<Real>() header, then the
+// attributes of every term in order.
+template <class Real, int... FORMS>
 struct RecordLayout
 {
     using FL = FormList<FORMS...>;
     static constexpr int attr( int t )
     {
-        int o = RHEAD;
+        int o = rhead<Real>();
         for ( int i = 0; i < t; ++i )
             o += np_of( FL::v[i] );
         return o;
@@ -725,25 +736,37 @@ struct PencilDofs
 };
 
 // One record's contribution to a lane's K x DT accumulator (nodes i of the
-// lane's (j, k) row).  The record holds the particle's x weights and weight
-// gradients (formed once when the record is written, instead of by each of
-// the task's K*K lanes), its y and z cell offsets and its premultiplied
-// attributes.
+// lane's (j, k) row).  The record holds the particle's header (rhead) and
+// its premultiplied attributes.
 template <class Real, bool SHARED, int... FORMS>
-__device__ __forceinline__ void pencil_record( const Real* rp, const NodePoly<Real>& py,
+__device__ __forceinline__ void pencil_record( const Real* rp, Real rdx0, const NodePoly<Real>& py,
                                                const NodePoly<Real>& pz,
                                                Real ( *acc )[SlotDofs<FORMS...>::total( SHARED )] )
 {
     using FL = FormList<FORMS...>;
     using SD = SlotDofs<FORMS...>;
-    using RL = RecordLayout<FORMS...>;
+    using RL = RecordLayout<Real, FORMS...>;
     using PD = PencilDofs<SHARED, FORMS...>;
     constexpr int DT = SD::total( SHARED );
-    const Real* wx = rp;
-    const Real* gx = rp + K;
+    const Real* wx;
+    const Real* gx;
     Real wy, gy, wz, gz;
-    py.eval( rp[2 * K], wy, gy );
-    pz.eval( rp[2 * K + 1], wz, gz );
+    Spline1D<ORDER, Real> sx;
+    if constexpr ( x_weights_in_record<Real>() )
+    {
+        wx = rp;
+        gx = rp + K;
+        py.eval( rp[2 * K], wy, gy );
+        pz.eval( rp[2 * K + 1], wz, gz );
+    }
+    else
+    {
+        spline_weights<ORDER, Real, false>( rp[0], rdx0, sx );
+        wx = sx.w;
+        gx = sx.g;
+        py.eval( rp[1], wy, gy );
+        pz.eval( rp[2], wz, gz );
+    }
     const Real A = wy * wz, B = gy * wz, C = wy * gz;
     Real al[DT], be[DT];
 #pragma unroll
@@ -796,10 +819,10 @@ template <class Real, bool CONTIG, int MAXM, int NTH, int... FORMS>
 __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileArgs& ta,
                                                const int64_t* run_start, const int* run_pre,
                                                int nruns, int pbeg, int nbatch,
-                                               PartPrefetch<Real, MAXM, RecordLayout<FORMS...>::len() - RHEAD>& pf )
+                                               PartPrefetch<Real, MAXM, RecordLayout<Real, FORMS...>::len() - rhead<Real>()>& pf )
 {
     using FL = FormList<FORMS...>;
-    using RL = RecordLayout<FORMS...>;
+    using RL = RecordLayout<Real, FORMS...>;
     // Run of every slot: count the run starts at or below its flattened
     // index (broadcast LDS reads, no dependent search).
     int run[MAXM];
@@ -827,7 +850,7 @@ __device__ __forceinline__ void prefetch_part( const ScatterArgs& a, const TileA
         static_for<FL::N>( [&]( auto I ) {
             constexpr int t = I;
             a.t[t].pin.template load<Real, np_of( FL::v[t] ), CONTIG>(
-                p, &pf.attr[mi][RL::attr( t ) - RHEAD] );
+                p, &pf.attr[mi][RL::attr( t ) - rhead<Real>()] );
         } );
     }
 }
@@ -853,14 +876,14 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
 {
     using FL = FormList<FORMS...>;
     using SD = SlotDofs<FORMS...>;
-    using RL = RecordLayout<FORMS...>;
+    using RL = RecordLayout<Real, FORMS...>;
     constexpr int NT = FL::N;
     constexpr int DT = SD::total( SHARED );
     constexpr int RLEN = RL::len();
     constexpr int RS = record_stride( RLEN, (int)sizeof( Real ) );
     constexpr int SBLOCK = sblock<Real>();
     constexpr int MAXM = SCATTER_SLOTS;
-    constexpr int NPT = RLEN - RHEAD;
+    constexpr int NPT = RLEN - rhead<Real>();
     constexpr int LMAX = SCATTER_TASK_MAX;
     // Pencil lanes: a task's K*K lanes own the (j, k) rows of its stencil.
     constexpr int KK = K * K;
@@ -932,6 +955,7 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
     const int pj = lg / K, pk = lg - ( lg / K ) * K;
     const bool pencil_lane = gw < GPW;
     const int gid = ( (int)threadIdx.x >> 6 ) * GPW + gw;
+    const Real rdx0 = (Real)a.mesh.rdx[0];
     NodePoly<Real> py, pz;
     py.init( pj, (Real)a.mesh.rdx[1] );
     pz.init( pk, (Real)a.mesh.rdx[2] );
@@ -1008,6 +1032,7 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
                                                                      a.mesh.rdx[d] ),
                                                a.mesh.num_node[d], bs[d], fr[d] ) &&
                      ok;
+            if constexpr ( x_weights_in_record<Real>() )
             {
                 Spline1D<ORDER, Real> sx;
                 spline_weights<ORDER, Real, false>( fr[0], (Real)a.mesh.rdx[0], sx );
@@ -1019,6 +1044,12 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
                 }
                 v[2 * K] = fr[1];
                 v[2 * K + 1] = fr[2];
+            }
+            else
+            {
+#pragma unroll
+                for ( int d = 0; d < 3; ++d )
+                    v[d] = fr[d];
             }
             if ( !ok )
             {
@@ -1043,7 +1074,7 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
                     const Real m = (Real)a.t[t].mult;
 #pragma unroll
                     for ( int c = 0; c < NP; ++c )
-                        v[RL::attr( t ) + c] = pf.attr[mi][RL::attr( t ) - RHEAD + c] * m;
+                        v[RL::attr( t ) + c] = pf.attr[mi][RL::attr( t ) - rhead<Real>() + c] * m;
                 } );
                 detail::vec_copy_out<Real, RLEN>( v, rec + (size_t)RS * i );
             }
@@ -1165,7 +1196,7 @@ __global__ __launch_bounds__( sblock<Real>(), scatter_min_waves<Real>() ) void s
                 {
                     Real cur[RLEN];
                     read_record<Real, RLEN>( rec + (size_t)RS * ord[r], cur );
-                    pencil_record<Real, SHARED, FORMS...>( cur, py, pz, acc );
+                    pencil_record<Real, SHARED, FORMS...>( cur, rdx0, py, pz, acc );
                 }
                 double* wb = win + ( ( lx * W[1] + ly + pj ) * W[2] + lz + pk ) * DT;
 #pragma unroll
