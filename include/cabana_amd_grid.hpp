@@ -793,6 +793,27 @@ void g2p_adjoint( const ArrayT& array, const HaloT& halo, const PointCoordinates
                    ctx );
 }
 
+// ---------------------------------------------------------------------------
+// Library switches (no reference counterpart)
+// ---------------------------------------------------------------------------
+
+// Deterministic P2G for the calls that follow (cabana_amd_set_deterministic):
+// every node sums its particles in a fixed order, so repeated calls give
+// bitwise equal grids; Node arrays only, much slower than the default.
+inline void setDeterministic( bool enable )
+{
+    auto* ctx = detail::context();
+    detail::check( cabana_amd_set_deterministic( ctx, enable ? 1 : 0 ), ctx );
+}
+
+// Whether calls without a binning bin the particles themselves and run the
+// cell-tiled kernels (cabana_amd_set_auto_binning; on by default).
+inline void setAutoBinning( bool enable )
+{
+    auto* ctx = detail::context();
+    detail::check( cabana_amd_set_auto_binning( ctx, enable ? 1 : 0 ), ctx );
+}
+
 } // namespace grid
 } // namespace cabana_amd
 

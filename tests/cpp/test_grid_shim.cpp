@@ -395,11 +395,58 @@ static void rebuild_test()
                  e, total / msum );
 }
 
+// Deterministic P2G through the C++ switch: three calls on the same inputs
+// give bitwise equal grids, and the grid agrees with the default path.
+static void deterministic_test()
+{
+    const int cells = 12, h = 1;
+    int n[3] = { cells, cells, cells };
+    GlobalMesh gm{ { 0.0, 0.0, 0.0 }, { 12.0, 12.0, 12.0 }, 1.0 };
+    auto lg = std::make_shared<LocalGrid>( LocalGrid{ GlobalGrid{ gm }, h } );
+    const std::size_t np = 30000;
+    std::mt19937_64 rng( 77 );
+    std::normal_distribution<double> Nd( 6.0, 1.5 ); // crowded cells near the middle
+    std::vector<double> ph( np * 3 ), mh( np );
+    for ( std::size_t p = 0; p < np; ++p )
+    {
+        for ( int d = 0; d < 3; ++d )
+            ph[3 * p + d] = std::fmin( std::fmax( Nd( rng ), 0.01 ), 11.99 );
+        mh[p] = 0.5 + 0.01 * double( p % 100 );
+    }
+    DeviceView<double, 2> points( { np, 3 } );
+    points.from_host( ph );
+    DeviceView<float, 1> mass( { np } );
+    mass.from_host( std::vector<float>( mh.begin(), mh.end() ) );
+    auto run = [&]() {
+        auto g = make_array<float>( lg, 1, n );
+        auto halo = cg::createHalo( cg::NodeHaloPattern<3>(), h, *g );
+        cg::p2g( cg::createScalarValueP2G( mass, 1.0 ), points, np, cg::Spline<2>(), *halo, *g );
+        return g->v.to_host();
+    };
+    const auto ref = run();
+    cg::setDeterministic( true );
+    const auto a = run();
+    const auto b = run();
+    const auto c = run();
+    cg::setDeterministic( false );
+    bool same = a.size() == b.size() && b.size() == c.size();
+    double err = 0.0, scale = 0.0;
+    for ( std::size_t i = 0; same && i < a.size(); ++i )
+    {
+        same = a[i] == b[i] && a[i] == c[i];
+        err = std::fmax( err, std::fabs( double( a[i] ) - double( ref[i] ) ) );
+        scale = std::fmax( scale, std::fabs( double( ref[i] ) ) );
+    }
+    EXPECT_NEAR( same ? 1.0 : 0.0, 1.0, 0.0 );
+    EXPECT_NEAR( err / scale, 0.0, 1e-5 );
+}
+
 int main()
 {
     reference_interpolation_test();
     call_forms_test();
     rebuild_test();
+    deterministic_test();
     HIP_OK( hipDeviceSynchronize() );
     std::printf( "%d checks, %d failures\n", g_checks, g_failures );
     if ( g_failures )
