@@ -744,7 +744,7 @@ bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int 
 {
     if ( kind == TILE_GATHER )
     {
-        static const int cand[] = { 8, 6, 4, 3, 2 };
+        static const int cand[] = { 8, 7, 6, 5, 4, 3, 2 };
         for ( int c : cand )
         {
             TileShape sh = { { c, c, c }, 0, 0 };
