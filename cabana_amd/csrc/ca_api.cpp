@@ -733,7 +733,8 @@ size_t tile_lds_bytes( TileKind kind, bool fp64, const int* slot_ng, int nslot,
 }
 
 // Brick shape and part capacity.  Gathers use the largest cubic brick whose
-// windows fit three workgroups per CU, with parts of ~1.25x the mean brick
+// windows fit 52 KB of LDS for FP32 (three workgroups per CU) and 78 KB for
+// FP64 (whose registers allow two), with parts of ~1.25x the mean brick
 // population.  Scatters sort a whole part in LDS: the brick is the largest
 // near-cubic one whose expected population (x1.2) fits a part (at most two
 // particles per thread) that, with the FP64 window, stays within 78 KB of
@@ -749,7 +750,7 @@ bool choose_shape( TileKind kind, bool fp64, const int* slot_ng, int nslot, int 
         {
             TileShape sh = { { c, c, c }, 0, 0 };
             if ( tile_lds_bytes( kind, fp64, slot_ng, nslot, rec_len, order, sh ) <=
-                 52 * 1024 )
+                 ( fp64 ? 78 : 52 ) * 1024 )
             {
                 const double mean = mean_ppc * c * c * c;
                 double cap = 1.25 * mean;
