@@ -1390,16 +1390,35 @@ cabana_amd_status p2g_impl( cabana_amd_context* ctx, const cabana_amd_mesh* mesh
         {
             CA_CHECK( build_library_bins( ctx, m, a.points, n, s ) );
             CA_HIP( ctx, run_count_outside( ctx, order, m, a.points, n, s ), "domain check" );
+            PullArgs pa;
+            std::memset( &pa, 0, sizeof( pa ) );
+            pa.mesh = m;
+            pa.points = a.points;
+            pa.counts = ctx->auto_bins.counts;
+            pa.offsets = ctx->auto_bins.offsets;
+            pa.perm = ctx->auto_perm;
+            // The MPM set (ScalarValue, VectorValue, TensorDivergence, in any
+            // call order) in one pass: each candidate particle's weights are
+            // formed once for the three terms.  Every other term alone.
+            std::vector<bool> taken( num_terms, false );
+            const int iSV = find_form( T, num_terms, FORM_SV, taken );
+            const int iVV = find_form( T, num_terms, FORM_VV, taken );
+            const int iTD = find_form( T, num_terms, FORM_TD, taken );
+            if ( iSV >= 0 && iVV >= 0 && iTD >= 0 )
+            {
+                pa.nterms = 3;
+                pa.term[0] = multi_term( T[iSV] );
+                pa.term[1] = multi_term( T[iVV] );
+                pa.term[2] = multi_term( T[iTD] );
+                taken[iSV] = taken[iVV] = taken[iTD] = true;
+                CA_HIP( ctx, run_pull( ctx, order, fp64, pa, s ), "p2g pull" );
+            }
             for ( int i = 0; i < num_terms; ++i )
             {
-                PullArgs pa;
-                std::memset( &pa, 0, sizeof( pa ) );
-                pa.mesh = m;
-                pa.points = a.points;
-                pa.term = multi_term( T[i] );
-                pa.counts = ctx->auto_bins.counts;
-                pa.offsets = ctx->auto_bins.offsets;
-                pa.perm = ctx->auto_perm;
+                if ( taken[i] )
+                    continue;
+                pa.nterms = 1;
+                pa.term[0] = multi_term( T[i] );
                 CA_HIP( ctx, run_pull( ctx, order, fp64, pa, s ), "p2g pull" );
             }
         }

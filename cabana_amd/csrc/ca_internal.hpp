@@ -323,7 +323,10 @@ struct PullArgs
 {
     MeshDev mesh;
     PFieldDev points;
-    MultiTermDev term;
+    // One term, or the MPM set (ScalarValue, VectorValue, TensorDivergence)
+    // in one pass; every term's grid has the same extents.
+    MultiTermDev term[3];
+    int32_t nterms;
     const int32_t* counts;
     const int64_t* offsets;
     const int64_t* perm;

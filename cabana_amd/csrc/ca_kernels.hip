@@ -545,13 +545,13 @@ __device__ __forceinline__ void pick_node( const Spline1D<ORDER, Real>& s, int j
     } );
 }
 
-template <class Real, int FORM>
+template <class Real, int... FORMS>
 __global__ __launch_bounds__( BLOCK ) void p2g_pull_kernel( PullArgs a )
 {
-    constexpr int NP = FormTraits<FORM>::NP;
-    constexpr int NG = FormTraits<FORM>::NG;
+    using FL = FormList<FORMS...>;
+    constexpr int NT = FL::N;
     const MeshDev& m = a.mesh;
-    const GFieldDev& g = a.term.grid;
+    const GFieldDev& g = a.term[0].grid;
     const int64_t total = (int64_t)g.ext[0] * g.ext[1] * g.ext[2];
     const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if ( e >= total )
@@ -573,11 +573,12 @@ __global__ __launch_bounds__( BLOCK ) void p2g_pull_kernel( PullArgs a )
         lo[d] = min( max( top - K, 0 ), N - 1 );
         hi[d] = min( max( top, 0 ), N - 1 );
     }
-    const Real mult = (Real)a.term.mult;
-    double acc[NG];
+    double acc[NT][3];
 #pragma unroll
-    for ( int c = 0; c < NG; ++c )
-        acc[c] = 0.0;
+    for ( int t = 0; t < NT; ++t )
+#pragma unroll
+        for ( int c = 0; c < 3; ++c )
+            acc[t][c] = 0.0;
     for ( int ci = lo[0]; ci <= hi[0]; ++ci )
         for ( int cj = lo[1]; cj <= hi[1]; ++cj )
             for ( int ck = lo[2]; ck <= hi[2]; ++ck )
@@ -620,23 +621,35 @@ __global__ __launch_bounds__( BLOCK ) void p2g_pull_kernel( PullArgs a )
                     nw.dW[0] = gr[0] * wyz;
                     nw.dW[1] = w[0] * gr[1] * w[2];
                     nw.dW[2] = w[0] * w[1] * gr[2];
-                    const uint32_t b = a.term.f.base( p );
-                    Real Pm[NP];
+                    static_for<NT>( [&]( auto I ) {
+                        constexpr int t = I;
+                        constexpr int F = FL::v[t];
+                        constexpr int NP = FormTraits<F>::NP;
+                        constexpr int NG = FormTraits<F>::NG;
+                        const Real mult = (Real)a.term[t].mult;
+                        const uint32_t b = a.term[t].f.base( p );
+                        Real Pm[NP];
 #pragma unroll
-                    for ( int c = 0; c < NP; ++c )
-                        Pm[c] = *a.term.f.template ptr<const Real>( b, c ) * mult;
-                    Real G[NG];
-                    form_scatter<FORM>( Pm, nw, G );
+                        for ( int c = 0; c < NP; ++c )
+                            Pm[c] = *a.term[t].f.template ptr<const Real>( b, c ) * mult;
+                        Real G[NG];
+                        form_scatter<F>( Pm, nw, G );
 #pragma unroll
-                    for ( int c = 0; c < NG; ++c )
-                        acc[c] += (double)G[c];
+                        for ( int c = 0; c < NG; ++c )
+                            acc[t][c] += (double)G[c];
+                    } );
                 }
             }
-    Real* dst = gptr<Real>( g, nd[0] * g.s[0] + nd[1] * g.s[1] + nd[2] * g.s[2] );
+    static_for<NT>( [&]( auto I ) {
+        constexpr int t = I;
+        constexpr int NG = FormTraits<FL::v[t]>::NG;
+        const GFieldDev& gt = a.term[t].grid;
+        Real* dst = gptr<Real>( gt, nd[0] * gt.s[0] + nd[1] * gt.s[1] + nd[2] * gt.s[2] );
 #pragma unroll
-    for ( int c = 0; c < NG; ++c )
-        if ( acc[c] != 0.0 )
-            dst[c * g.s[3]] += (Real)acc[c];
+        for ( int c = 0; c < NG; ++c )
+            if ( acc[t][c] != 0.0 )
+                dst[c * gt.s[3]] += (Real)acc[t][c];
+    } );
 }
 
 // Particles whose stencil leaves the ghosted grid (counted once each, as
@@ -660,11 +673,23 @@ __global__ __launch_bounds__( BLOCK ) void count_outside_kernel( MeshDev m, PFie
 template <class Real>
 hipError_t pull_typed( Launcher& L, const PullArgs& a, hipStream_t s )
 {
-    const int64_t total = (int64_t)a.term.grid.ext[0] * a.term.grid.ext[1] * a.term.grid.ext[2];
+    const int64_t total =
+        (int64_t)a.term[0].grid.ext[0] * a.term[0].grid.ext[1] * a.term[0].grid.ext[2];
     if ( total <= 0 )
         return hipSuccess;
-    L.before( "p2g_pull", s );
-    switch ( a.term.form )
+    L.before( a.nterms == 3 ? "p2g_pull_mpm" : "p2g_pull", s );
+    if ( a.nterms == 3 )
+    {
+        if ( a.term[0].form != FORM_SV || a.term[1].form != FORM_VV || a.term[2].form != FORM_TD )
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL( ( p2g_pull_kernel<Real, FORM_SV, FORM_VV, FORM_TD> ),
+                            blocks_for( total ), dim3( BLOCK ), 0, s, a );
+        L.after( "p2g_pull_mpm", s );
+        return hipGetLastError();
+    }
+    if ( a.nterms != 1 )
+        return hipErrorInvalidValue;
+    switch ( a.term[0].form )
     {
 #define CA_CASE( F )                                                           \
     case F:                                                                    \
