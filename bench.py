@@ -14,7 +14,7 @@ build + permute (inputs resident in HBM before the timed region); the
 per-step sort is timed as a separate figure.  The headline value is the
 FP32-mixed forward+adjoint rate; the FP64 plain path runs the same step.
 
-Prints ONE JSON line.  --gpus N > 1 exits: this build measures one GPU.
+Prints ONE JSON line.  --gpus N > 1 exits: the benchmark measures one GPU.
 """
 import argparse
 import json
@@ -580,7 +580,7 @@ def main():
                   [int(n) for n in args.cpu_sizes.split(",")], runs=3)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only (this build has no multi-GPU path); "
+        print("bench.py measures one GPU only (the library has no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     # The CPU baseline runs first, in a child process, before this process
