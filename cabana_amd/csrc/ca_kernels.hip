@@ -586,11 +586,24 @@ __global__ __launch_bounds__( BLOCK ) void p2g_pull_kernel( PullArgs a )
                 const int64_t cell = ( (int64_t)ci * m.num_cell[1] + cj ) * m.num_cell[2] + ck;
                 const int64_t q0 = a.offsets[cell];
                 const int64_t q1 = q0 + a.counts[cell];
+                // The next candidate's index and position are read while
+                // this one is tested and added.
+                int64_t pn = 0;
+                double pxn[3] = { 0.0, 0.0, 0.0 };
+                if ( q0 < q1 )
+                {
+                    pn = a.perm[q0];
+                    load_position( a.points, pn, pxn );
+                }
                 for ( int64_t q = q0; q < q1; ++q )
                 {
-                    const int64_t p = a.perm[q];
-                    double px[3];
-                    load_position( a.points, p, px );
+                    const int64_t p = pn;
+                    double px[3] = { pxn[0], pxn[1], pxn[2] };
+                    if ( q + 1 < q1 )
+                    {
+                        pn = a.perm[q + 1];
+                        load_position( a.points, pn, pxn );
+                    }
                     // Stencil test first (most candidates miss the node),
                     // weights only for the particles that reach it.
                     Real f[3];
