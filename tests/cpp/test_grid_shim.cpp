@@ -441,12 +441,86 @@ static void deterministic_test()
     EXPECT_NEAR( err / scale, 0.0, 1e-5 );
 }
 
+// adjoint_gather of the MPM set (one fused pass) against p2g_adjoint and
+// g2p_adjoint called one after the other.
+static void adjoint_gather_test()
+{
+    const int cells = 10, h = 1;
+    int n[3] = { cells, cells, cells };
+    GlobalMesh gm{ { 0.0, 0.0, 0.0 }, { 10.0, 10.0, 10.0 }, 1.0 };
+    auto lg = std::make_shared<LocalGrid>( LocalGrid{ GlobalGrid{ gm }, h } );
+    const std::size_t np = 8000;
+    std::mt19937_64 rng( 91 );
+    std::uniform_real_distribution<double> U( 0.0, 10.0 );
+    std::normal_distribution<double> Nd( 0.0, 1.0 );
+    auto rnd = [&]( std::size_t k, bool pos ) {
+        std::vector<double> v( k );
+        for ( auto& x : v )
+            x = pos ? U( rng ) : Nd( rng );
+        return v;
+    };
+    DeviceView<double, 2> points( { np, 3 } );
+    points.from_host( rnd( 3 * np, true ) );
+    DeviceView<double, 1> m( { np } );
+    DeviceView<double, 2> q( { np, 3 } ), uv( { np, 3 } );
+    DeviceView<double, 3> sig( { np, 3, 3 } ), ul( { np, 3, 3 } );
+    m.from_host( rnd( np, false ) );
+    q.from_host( rnd( 3 * np, false ) );
+    uv.from_host( rnd( 3 * np, false ) );
+    sig.from_host( rnd( 9 * np, false ) );
+    ul.from_host( rnd( 9 * np, false ) );
+    auto gm_bar = make_array<double>( lg, 1, n );
+    auto gq_bar = make_array<double>( lg, 3, n );
+    auto gs_bar = make_array<double>( lg, 3, n );
+    auto u = make_array<double>( lg, 3, n );
+    for ( auto* a : { gm_bar.get(), gq_bar.get(), gs_bar.get(), u.get() } )
+        a->v.from_host( rnd( a->v.size(), false ) );
+    auto halo = cg::createHalo( cg::NodeHaloPattern<3>(), h, *u );
+    auto fm = cg::createScalarValueP2G( m, 1.0 );
+    auto fq = cg::createVectorValueP2G( q, 1.0 );
+    auto fs = cg::createTensorDivergenceP2G( sig, -0.5 );
+    auto gv = cg::createVectorValueG2P( uv, 0.7 );
+    auto gl = cg::createVectorGradientG2P( ul, -1.1 );
+
+    // Fused.
+    DeviceView<double, 1> mb( { np } );
+    DeviceView<double, 2> qb( { np, 3 } ), xb( { np, 3 } );
+    DeviceView<double, 3> sb( { np, 3, 3 } );
+    cg::adjoint_gather( { cg::bind_adjoint( fm, *gm_bar, mb ), cg::bind_adjoint( fq, *gq_bar, qb ),
+                          cg::bind_adjoint( fs, *gs_bar, sb ) },
+                        *u, *halo, points, np, cg::Spline<2>(), { gv, gl }, xb );
+    // Separate calls.
+    DeviceView<double, 1> mb2( { np } );
+    DeviceView<double, 2> qb2( { np, 3 } ), xb2( { np, 3 } );
+    DeviceView<double, 3> sb2( { np, 3, 3 } );
+    cg::p2g_adjoint( fm, points, np, cg::Spline<2>(), *halo, *gm_bar, mb2, &xb2 );
+    cg::p2g_adjoint( fq, points, np, cg::Spline<2>(), *halo, *gq_bar, qb2, &xb2 );
+    cg::p2g_adjoint( fs, points, np, cg::Spline<2>(), *halo, *gs_bar, sb2, &xb2 );
+    auto scratch = make_array<double>( lg, 3, n );
+    cg::g2p_adjoint( *u, *halo, points, np, cg::Spline<2>(), gv, *scratch, &xb2 );
+    cg::g2p_adjoint( *u, *halo, points, np, cg::Spline<2>(), gl, *scratch, &xb2 );
+    auto close = []( const std::vector<double>& a, const std::vector<double>& b ) {
+        double err = 0.0, scale = 0.0;
+        for ( std::size_t i = 0; i < a.size(); ++i )
+        {
+            err = std::fmax( err, std::fabs( a[i] - b[i] ) );
+            scale = std::fmax( scale, std::fabs( b[i] ) );
+        }
+        return err / std::fmax( scale, 1e-300 );
+    };
+    EXPECT_NEAR( close( mb.to_host(), mb2.to_host() ), 0.0, 1e-13 );
+    EXPECT_NEAR( close( qb.to_host(), qb2.to_host() ), 0.0, 1e-13 );
+    EXPECT_NEAR( close( sb.to_host(), sb2.to_host() ), 0.0, 1e-13 );
+    EXPECT_NEAR( close( xb.to_host(), xb2.to_host() ), 0.0, 1e-12 );
+}
+
 int main()
 {
     reference_interpolation_test();
     call_forms_test();
     rebuild_test();
     deterministic_test();
+    adjoint_gather_test();
     HIP_OK( hipDeviceSynchronize() );
     std::printf( "%d checks, %d failures\n", g_checks, g_failures );
     if ( g_failures )
