@@ -375,11 +375,11 @@ __host__ __device__ constexpr int record_stride( int len, int elem_bytes )
 // Threads of a scatter workgroup (FP32 / FP64 records) and the particles
 // each thread holds per part; their product is the largest part.
 constexpr int scatter_block( bool fp64 ) { return 256; }
-constexpr int SCATTER_SLOTS = 2;
+constexpr int SCATTER_SLOTS = 2; // tested value; 3 failed the unbinned tests
 // Longest run of one base cell's records that one scatter task takes
 // (8 measured faster than 16 or 32 at 16 particles per cell: tasks of more
 // even length leave fewer lanes idle).
-constexpr int SCATTER_TASK_MAX = 8;
+constexpr int SCATTER_TASK_MAX = 8; // tested value; 12 failed the dense-cell tests
 constexpr int scatter_cap_max( bool fp64 ) { return scatter_block( fp64 ) * SCATTER_SLOTS; }
 
 // Launch-time error/timing context owned by the C-ABI layer.
